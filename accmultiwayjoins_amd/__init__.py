@@ -1,0 +1,9 @@
+"""MI355X-native multi-way joins (AMHJ hash join and ALFTJ leapfrog triejoin).
+
+The compute path is libamj.so (hand-written gfx950 HIP kernels behind a C ABI, see
+include/amj.h); this package is its host-side interface.
+"""
+from .amj import AmjError, Context, HashTable, JoinResult, Relation  # noqa: F401
+from . import generators  # noqa: F401
+
+__all__ = ["AmjError", "Context", "HashTable", "JoinResult", "Relation", "generators"]

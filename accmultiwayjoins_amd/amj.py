@@ -1,0 +1,223 @@
+"""Host-side operator interface of the MI355X multi-way joins, over libamj's C ABI.
+
+Mirrors the reference's operator boundary:
+  * ``Context.chain_join``   -> exec-AMHJ NORMAL (MHJ-GPU/exec_AMHJ.cu:70-89, :163-203): the
+    first relation probes, every other relation is hashed on its first column
+    (QueryProcessor::build_hash_multiway, MHJ-GPU/QueryProcessor.cuh:76-110);
+  * ``Context.pattern_join`` -> exec-AMHJ TRI / FOUR on one edge relation
+    (MHJ-GPU/exec_AMHJ.cu:32-69, QueryProcessor::build_hash_single :112-169);
+  * ``Context.execute``      -> OOC<...>::execute (MHJ-GPU/ooc.cuh:77-82) on resident relations
+    and hash tables (``Context.relation`` / ``Context.hash_table``).
+Switches keep the reference's names: ooc, ws, dro (the reference's `sbp` argument), fib.
+Errors raise ``AmjError`` carrying the library's message instead of exiting the process.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _native as N
+
+
+class AmjError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{N.STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+@dataclass
+class JoinResult:
+    count: int
+    checksum: int
+    attr_ids: list
+    path: str
+    build_ms: float
+    probe_ms: float
+    probe_kernel_ms: float
+    cols: Optional[list] = None  # numpy int32 columns when fetched
+    rows_written: int = 0
+    _res: Optional[N.AmjResult] = field(default=None, repr=False)
+
+    def tuples(self) -> np.ndarray:
+        """Materialised result as an (count, n_attrs) array (columns in attr_ids order)."""
+        if self.cols is None:
+            raise ValueError("result was not fetched")
+        if not self.cols:
+            return np.zeros((0, 0), dtype=np.int32)
+        return np.stack(self.cols, axis=1)
+
+
+def _opts(ooc=0, ws=0, dro=0, fib=0, materialize=True, hbm_budget_bytes=0, ooc_slice_rows=0,
+          out_chunk_rows=0, dro_capacity_rows=0, ws_split_threshold=0) -> N.AmjOpts:
+    o = N.AmjOpts()
+    N.lib().amj_default_opts(C.byref(o))
+    o.ooc, o.ws, o.dro, o.fib = int(ooc), int(ws), int(dro), int(fib)
+    o.materialize = 1 if materialize else 0
+    o.hbm_budget_bytes = int(hbm_budget_bytes)
+    o.ooc_slice_rows = int(ooc_slice_rows)
+    o.out_chunk_rows = int(out_chunk_rows)
+    o.dro_capacity_rows = int(dro_capacity_rows)
+    o.ws_split_threshold = int(ws_split_threshold)
+    return o
+
+
+class _RelKeep:
+    """Keeps the ctypes arrays behind an AmjRel alive."""
+
+    def __init__(self, cols: Sequence[np.ndarray], attr_ids: Optional[Sequence[int]]):
+        self.arrays = [np.ascontiguousarray(c, dtype=np.int32) for c in cols]
+        rows = len(self.arrays[0]) if self.arrays else 0
+        if any(len(a) != rows for a in self.arrays):
+            raise ValueError("columns of a relation must have equal length")
+        self.ptrs = (C.c_void_p * len(self.arrays))(*[a.ctypes.data for a in self.arrays])
+        self.ids = None
+        if attr_ids is not None:
+            self.ids = (C.c_uint32 * len(attr_ids))(*[int(x) for x in attr_ids])
+        self.rel = N.AmjRel()
+        self.rel.num_cols = len(self.arrays)
+        self.rel.rows = rows
+        self.rel.cols = C.cast(self.ptrs, C.POINTER(C.c_void_p))
+        self.rel.attr_ids = C.cast(self.ids, C.POINTER(C.c_uint32)) if self.ids else None
+        self.rel.on_device = 0
+
+
+class Relation:
+    """A device-resident relation (Relation<D,C>, MHJ-GPU/Relation.cuh:9-48)."""
+
+    def __init__(self, ctx: "Context", cols, attr_ids):
+        keep = _RelKeep(cols, attr_ids)
+        h = C.c_void_p()
+        ctx._check(N.lib().amj_relation_create(ctx.h, C.byref(keep.rel), C.byref(h)))
+        self.ctx, self.h, self.rows = ctx, h, keep.rel.rows
+
+    def close(self):
+        if self.h:
+            N.lib().amj_relation_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class HashTable:
+    """Bucketed hash table over a relation (HashTable<D,C>, MHJ-GPU/Relation.cuh:50-77)."""
+
+    def __init__(self, ctx: "Context", rel: Relation, key_col: int = 0, sort_vals: bool = False):
+        h = C.c_void_p()
+        ctx._check(N.lib().amj_hash_table_build(ctx.h, rel.h, key_col, int(sort_vals),
+                                                C.byref(h)))
+        self.ctx, self.h, self.rel = ctx, h, rel
+
+    def close(self):
+        if self.h:
+            N.lib().amj_hash_table_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Context:
+    """One GPU, one HIP stream. Not shared between host threads."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        rc = N.lib().amj_ctx_create(device, C.byref(h))
+        if rc != N.AMJ_OK:
+            raise AmjError(rc, "cannot create a context on device %d" % device)
+        self.h = h
+
+    # -- plumbing ---------------------------------------------------------------------------
+    def _check(self, rc: int):
+        if rc != N.AMJ_OK:
+            msg = N.lib().amj_last_error(self.h)
+            raise AmjError(rc, msg.decode() if msg else "")
+
+    def close(self):
+        if getattr(self, "h", None):
+            N.lib().amj_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def device_info(self):
+        name = C.create_string_buffer(256)
+        hbm = C.c_uint64()
+        cus = C.c_int()
+        self._check(N.lib().amj_device_info(self.h, name, 256, C.byref(hbm), C.byref(cus)))
+        return name.value.decode(), hbm.value, cus.value
+
+    def _finish(self, res: N.AmjResult, fetch: bool) -> JoinResult:
+        out = JoinResult(
+            count=res.count, checksum=res.checksum,
+            attr_ids=[res.attr_ids[i] for i in range(res.num_cols)],
+            path=N.PATHS.get(res.path, str(res.path)), build_ms=res.build_ms,
+            probe_ms=res.probe_ms, probe_kernel_ms=res.probe_kernel_ms,
+            rows_written=res.rows_written, _res=res)
+        if fetch and res.rows_written:
+            cols = []
+            for c in range(res.num_cols):
+                a = np.empty(res.rows_written, dtype=np.int32)
+                self._check(N.lib().amj_result_copy_col(self.h, C.byref(res), c, 0,
+                                                        res.rows_written, a.ctypes.data))
+                cols.append(a)
+            out.cols = cols
+        elif fetch:
+            out.cols = [np.empty(0, dtype=np.int32) for _ in range(res.num_cols)]
+        if fetch:
+            self.free(out)
+        return out
+
+    def free(self, r: JoinResult):
+        if r._res is not None:
+            self._check(N.lib().amj_result_free(self.h, C.byref(r._res)))
+            r._res = None
+
+    # -- operators --------------------------------------------------------------------------
+    def chain_join(self, relations, fetch: bool = False, **opts) -> JoinResult:
+        """exec-AMHJ NORMAL. relations: [(columns, attr_ids), ...], relations[0] probes."""
+        keeps = [_RelKeep(cols, ids) for cols, ids in relations]
+        arr = (N.AmjRel * len(keeps))(*[k.rel for k in keeps])
+        o = _opts(**opts)
+        res = N.AmjResult()
+        self._check(N.lib().amj_chain_join(self.h, arr, len(keeps), C.byref(o), C.byref(res)))
+        return self._finish(res, fetch)
+
+    def pattern_join(self, src, dst, query: str = "TRI", fetch: bool = False,
+                     **opts) -> JoinResult:
+        """exec-AMHJ TRI / FOUR over one edge relation E(src, dst)."""
+        keep = _RelKeep([src, dst], [0, 1])
+        o = _opts(**opts)
+        res = N.AmjResult()
+        self._check(N.lib().amj_pattern_join(self.h, C.byref(keep.rel), N.QUERY[query],
+                                             C.byref(o), C.byref(res)))
+        return self._finish(res, fetch)
+
+    def relation(self, cols, attr_ids) -> Relation:
+        return Relation(self, cols, attr_ids)
+
+    def hash_table(self, rel: Relation, key_col: int = 0, sort_vals: bool = False) -> HashTable:
+        return HashTable(self, rel, key_col, sort_vals)
+
+    def execute(self, probe: Relation, tables: Sequence[HashTable], fetch: bool = False,
+                **opts) -> JoinResult:
+        """OOC::execute over resident relations and hash tables."""
+        hts = (C.c_void_p * len(tables))(*[t.h.value for t in tables])
+        o = _opts(**opts)
+        res = N.AmjResult()
+        self._check(N.lib().amj_execute(self.h, probe.h, hts, len(tables), C.byref(o),
+                                        C.byref(res)))
+        return self._finish(res, fetch)

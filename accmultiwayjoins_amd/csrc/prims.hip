@@ -1,0 +1,292 @@
+// Device-wide primitives written for gfx950: exclusive scan (reduce-then-scan), u64
+// reduction and a stable LSD radix sort of 64-bit keys with wave64 ballot multisplit.
+//
+// These replace the reference's uses of cub::DeviceScan::ExclusiveSum, DeviceReduce::Sum and
+// DeviceRadixSort::SortPairs (common-utils/cuda/primitives.cuh:258-510); the algorithms
+// are the textbook ones (Merrill & Garland's reduce-then-scan, LSD radix sort), written
+// from scratch for 64-lane waves.
+#include "amj_internal.h"
+
+namespace amj {
+
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;  // 2048 elements per workgroup
+
+size_t scan_temp_elems(uint64_t n) { return (size_t)((n + kScanTile - 1) / kScanTile) + 1; }
+
+template <typename T>
+__device__ __forceinline__ T wave_incl_scan(T x) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int d = 1; d < kWave; d <<= 1) {
+    T y = __shfl_up(x, (unsigned)d, kWave);
+    if (lane >= d) x += y;
+  }
+  return x;
+}
+
+// Block-wide exclusive scan of one value per thread; returns the block total too.
+template <typename T>
+__device__ __forceinline__ T block_excl_scan(T v, T* total, T* lds_wave) {
+  const int lane = lane_id();
+  const int w = threadIdx.x / kWave;
+  T incl = wave_incl_scan<T>(v);
+  if (lane == kWave - 1) lds_wave[w] = incl;
+  __syncthreads();
+  T wave_off = 0, tot = 0;
+#pragma unroll
+  for (int i = 0; i < kWavesPerBlock; ++i) {
+    T s = lds_wave[i];
+    if (i < w) wave_off += s;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return wave_off + incl - v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void scan_reduce_kernel(const T* __restrict__ in,
+                                                             uint64_t n, T* __restrict__ part) {
+  __shared__ T lds[kWavesPerBlock];
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+  T sum = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    uint64_t idx = base + (uint64_t)i * kBlock + threadIdx.x;
+    if (idx < n) sum += in[idx];
+  }
+  T total;
+  (void)block_excl_scan<T>(sum, &total, lds);
+  if (threadIdx.x == 0) part[blockIdx.x] = total;
+}
+
+// Single workgroup: exclusive scan of the per-tile partials, in place.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void scan_partials_kernel(T* part, uint64_t m) {
+  __shared__ T lds[kWavesPerBlock];
+  T carry = 0;
+  for (uint64_t base = 0; base < m; base += kScanTile) {
+    T v[kScanItems];
+    T sum = 0;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+      uint64_t idx = base + (uint64_t)threadIdx.x * kScanItems + i;
+      v[i] = idx < m ? part[idx] : (T)0;
+      sum += v[i];
+    }
+    T total;
+    T off = block_excl_scan<T>(sum, &total, lds) + carry;
+#pragma unroll
+    for (int i = 0; i < kScanItems; ++i) {
+      uint64_t idx = base + (uint64_t)threadIdx.x * kScanItems + i;
+      if (idx < m) part[idx] = off;
+      off += v[i];
+    }
+    carry += total;
+  }
+  if (threadIdx.x == 0) part[m] = carry;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void scan_down_kernel(const T* __restrict__ in, T* out,
+                                                           uint64_t n,
+                                                           const T* __restrict__ part) {
+  __shared__ T lds[kWavesPerBlock];
+  __shared__ T tile[kScanTile];
+  const uint64_t base = (uint64_t)blockIdx.x * kScanTile;
+  // striped coalesced load into LDS, blocked scan out of LDS
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    uint64_t idx = base + (uint64_t)i * kBlock + threadIdx.x;
+    tile[i * kBlock + threadIdx.x] = idx < n ? in[idx] : (T)0;
+  }
+  __syncthreads();
+  T v[kScanItems];
+  T sum = 0;
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    v[i] = tile[threadIdx.x * kScanItems + i];
+    sum += v[i];
+  }
+  T total;
+  T off = block_excl_scan<T>(sum, &total, lds) + part[blockIdx.x];
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    tile[threadIdx.x * kScanItems + i] = off;
+    off += v[i];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < kScanItems; ++i) {
+    uint64_t idx = base + (uint64_t)i * kBlock + threadIdx.x;
+    if (idx < n) out[idx] = tile[i * kBlock + threadIdx.x];
+  }
+}
+
+template <typename T>
+static hipError_t exclusive_scan_t(const T* in, T* out, uint64_t n, T* temp, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const uint64_t tiles = (n + kScanTile - 1) / kScanTile;
+  hipLaunchKernelGGL(scan_reduce_kernel<T>, dim3((unsigned)tiles), dim3(kBlock), 0, s, in, n,
+                     temp);
+  hipLaunchKernelGGL(scan_partials_kernel<T>, dim3(1), dim3(kBlock), 0, s, temp, tiles);
+  hipLaunchKernelGGL(scan_down_kernel<T>, dim3((unsigned)tiles), dim3(kBlock), 0, s, in, out,
+                     n, temp);
+  return hipGetLastError();
+}
+
+hipError_t exclusive_scan_u32(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* temp,
+                              hipStream_t s) {
+  return exclusive_scan_t<uint32_t>(in, out, n, temp, s);
+}
+hipError_t exclusive_scan_u64(const uint64_t* in, uint64_t* out, uint64_t n, uint64_t* temp,
+                              hipStream_t s) {
+  return exclusive_scan_t<uint64_t>(in, out, n, temp, s);
+}
+
+hipError_t reduce_sum_u64(const uint64_t* in, uint64_t n, uint64_t* out, uint64_t* temp,
+                          hipStream_t s) {
+  if (n == 0) return hipMemsetAsync(out, 0, sizeof(uint64_t), s);
+  const uint64_t tiles = (n + kScanTile - 1) / kScanTile;
+  hipLaunchKernelGGL(scan_reduce_kernel<uint64_t>, dim3((unsigned)tiles), dim3(kBlock), 0, s,
+                     in, n, temp);
+  hipLaunchKernelGGL(scan_partials_kernel<uint64_t>, dim3(1), dim3(kBlock), 0, s, temp, tiles);
+  return hipMemcpyAsync(out, temp + tiles, sizeof(uint64_t), hipMemcpyDeviceToDevice, s);
+}
+
+// ---------------------------------------------------------------------------------------
+// Radix sort
+// ---------------------------------------------------------------------------------------
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kBlock * kSortItems;  // 4096 keys per workgroup
+constexpr int kRadix = 256;
+
+static inline uint64_t sort_tiles(uint64_t n) { return (n + kSortTile - 1) / kSortTile; }
+
+size_t radix_sort_temp_bytes(uint64_t n) {
+  const uint64_t tiles = sort_tiles(n);
+  const uint64_t hist = (uint64_t)kRadix * tiles;
+  return 256 + sizeof(uint32_t) * (hist + 1) + sizeof(uint32_t) * scan_temp_elems(hist) + 256;
+}
+
+// OR over all keys of (key ^ key[0]): the bits in which the keys differ.
+__global__ __launch_bounds__(kBlock) void key_diff_kernel(const uint64_t* __restrict__ keys,
+                                                          uint64_t n, uint64_t* diff) {
+  const uint64_t k0 = keys[0];
+  uint64_t acc = 0;
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * kBlock)
+    acc |= keys[i] ^ k0;
+#pragma unroll
+  for (int d = kWave / 2; d >= 1; d >>= 1) acc |= __shfl_xor(acc, d, kWave);
+  if (lane_id() == 0 && acc) atomicOr((unsigned long long*)diff, (unsigned long long)acc);
+}
+
+__global__ __launch_bounds__(kBlock) void radix_hist_kernel(const uint64_t* __restrict__ keys,
+                                                            uint64_t n, int shift,
+                                                            uint32_t* __restrict__ hist,
+                                                            uint64_t tiles) {
+  __shared__ uint32_t cnt[kRadix];
+  cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+#pragma unroll 4
+  for (int i = 0; i < kSortItems; ++i) {
+    uint64_t idx = base + (uint64_t)i * kBlock + threadIdx.x;
+    if (idx < n) atomicAdd(&cnt[(uint32_t)(keys[idx] >> shift) & 0xffu], 1u);
+  }
+  __syncthreads();
+  hist[(uint64_t)threadIdx.x * tiles + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// Stable scatter: elements are ranked in (round, wave, lane) order, which is the input order
+// because element (round r, thread t) is key[tile*4096 + r*256 + t].
+__global__ __launch_bounds__(kBlock) void radix_scatter_kernel(
+    const uint64_t* __restrict__ keys, uint64_t* __restrict__ out, uint64_t n, int shift,
+    const uint32_t* __restrict__ hist_scanned, uint64_t tiles) {
+  __shared__ uint32_t running[kRadix];
+  __shared__ uint32_t wcnt[kWavesPerBlock][kRadix];
+  const int tid = threadIdx.x;
+  const int w = tid / kWave;
+  running[tid] = hist_scanned[(uint64_t)tid * tiles + blockIdx.x];
+#pragma unroll
+  for (int i = 0; i < kWavesPerBlock; ++i) wcnt[i][tid] = 0;
+  __syncthreads();
+  const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+  for (int r = 0; r < kSortItems; ++r) {
+    const uint64_t idx = base + (uint64_t)r * kBlock + tid;
+    const bool valid = idx < n;
+    const uint64_t key = valid ? keys[idx] : 0ull;
+    const uint32_t d = (uint32_t)(key >> shift) & 0xffu;
+    uint64_t m = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const bool bit = (d >> b) & 1u;
+      const uint64_t bb = __ballot(bit);
+      m &= bit ? bb : ~bb;
+    }
+    const uint32_t rank = mbcnt(m);
+    if (valid && rank == 0) wcnt[w][d] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = running[d] + rank;
+#pragma unroll
+      for (int i = 0; i < kWavesPerBlock; ++i)
+        if (i < w) pos += wcnt[i][d];
+      out[pos] = key;
+    }
+    __syncthreads();
+    uint32_t add = 0;
+#pragma unroll
+    for (int i = 0; i < kWavesPerBlock; ++i) {
+      add += wcnt[i][tid];
+      wcnt[i][tid] = 0;
+    }
+    running[tid] += add;
+    __syncthreads();
+  }
+}
+
+hipError_t radix_sort_u64(uint64_t* keys, uint64_t* alt, uint64_t n, uint32_t digit_mask,
+                          void* temp, hipStream_t s) {
+  if (n <= 1) return hipSuccess;
+  if (n >= (1ull << 32)) return hipErrorInvalidValue;
+  const uint64_t tiles = sort_tiles(n);
+  const uint64_t hist_n = (uint64_t)kRadix * tiles;
+  char* t = (char*)temp;
+  uint64_t* diff = (uint64_t*)t;
+  uint32_t* hist = (uint32_t*)(t + 256);
+  uint32_t* scan_tmp = hist + hist_n + 1;
+
+  hipError_t e = hipMemsetAsync(diff, 0, sizeof(uint64_t), s);
+  if (e != hipSuccess) return e;
+  const unsigned grid = (unsigned)std::min<uint64_t>((n + kBlock - 1) / kBlock, 4096);
+  hipLaunchKernelGGL(key_diff_kernel, dim3(grid), dim3(kBlock), 0, s, keys, n, diff);
+  uint64_t hdiff = 0;
+  e = hipMemcpyAsync(&hdiff, diff, sizeof(uint64_t), hipMemcpyDeviceToHost, s);
+  if (e != hipSuccess) return e;
+  e = hipStreamSynchronize(s);
+  if (e != hipSuccess) return e;
+
+  uint64_t* src = keys;
+  uint64_t* dst = alt;
+  for (int d = 0; d < 8; ++d) {
+    if (!((digit_mask >> d) & 1u)) continue;
+    if (((hdiff >> (8 * d)) & 0xffull) == 0) continue;  // digit constant: pass is identity
+    const int shift = 8 * d;
+    hipLaunchKernelGGL(radix_hist_kernel, dim3((unsigned)tiles), dim3(kBlock), 0, s, src, n,
+                       shift, hist, tiles);
+    e = exclusive_scan_u32(hist, hist, hist_n, scan_tmp, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(radix_scatter_kernel, dim3((unsigned)tiles), dim3(kBlock), 0, s, src,
+                       dst, n, shift, hist, tiles);
+    std::swap(src, dst);
+  }
+  if (src != keys)
+    e = hipMemcpyAsync(keys, src, n * sizeof(uint64_t), hipMemcpyDeviceToDevice, s);
+  if (e != hipSuccess) return e;
+  return hipGetLastError();
+}
+
+}  // namespace amj

@@ -1,0 +1,106 @@
+// Host-side runtime objects of libamj (context, relations, hash tables, device buffers).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "amj_internal.h"
+
+struct amj_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipStream_t side = nullptr;  // copy stream for out-of-core streaming
+  std::string last_error;
+  hipDeviceProp_t props{};
+  uint64_t allocated = 0;      // bytes currently allocated by this context
+  uint64_t peak = 0;
+  hipEvent_t ev[8]{};
+
+  // Reusable scratch buffers keyed by purpose.
+  struct Buf {
+    void* p = nullptr;
+    size_t bytes = 0;
+  };
+  std::map<int, Buf> scratch;
+  // Parked result columns, reused by the next result that fits.
+  std::vector<Buf> out_pool;
+};
+
+struct amj_relation {
+  uint32_t num_cols = 0;
+  uint64_t rows = 0;
+  int32_t* cols[AMJ_MAX_PROBE_COLS] = {nullptr, nullptr};
+  uint32_t attr_ids[AMJ_MAX_PROBE_COLS] = {0, 0};
+  bool owned = false;
+};
+
+struct amj_hash_table {
+  amj::HashTableDev dev;
+  const amj_relation* rel = nullptr;
+  uint32_t key_col = 0;
+  bool vals_sorted = false;
+  double build_ms = 0.0;
+};
+
+namespace amj {
+
+enum ScratchSlot {
+  SLOT_SORT_KEYS = 1,
+  SLOT_SORT_ALT,
+  SLOT_SORT_TEMP,
+  SLOT_UNIT_COUNTS,
+  SLOT_SCAN_TEMP,
+  SLOT_BLOCK_SUMS,
+  SLOT_SUM,
+  SLOT_FLAG,
+  SLOT_PROBE_OFF,
+  SLOT_PW0,  // SLOT_PW0 + t for level t
+  SLOT_PW_END = SLOT_PW0 + kMaxLevels,
+  SLOT_FIB_W,
+  SLOT_OOC_A,
+  SLOT_OOC_B,
+  SLOT_MISC0,
+  SLOT_MISC1,
+  SLOT_MISC2,
+  SLOT_MISC3,
+  SLOT_TRIE0,
+  SLOT_TRIE1,
+  SLOT_TRIE2,
+  SLOT_TRIE3,
+};
+
+int dmalloc(amj_ctx_t* ctx, void** p, size_t bytes);
+void dfree(amj_ctx_t* ctx, void* p, size_t bytes);
+int scratch(amj_ctx_t* ctx, int slot, size_t bytes, void** out);
+
+// A table as used by one query: the same physical table may be used several times with
+// different attribute ids (the TRI and FOUR queries alias one edge table).
+struct TableUse {
+  const amj_hash_table* ht;
+  uint32_t key_attr_id;
+  int64_t val_attr_id;  // -1: single-column table
+};
+
+struct ProbeUse {
+  const amj_relation* rel;
+  uint32_t attr_ids[AMJ_MAX_PROBE_COLS];
+};
+
+// Resolve attribute numbering and per-level modes; reorders compare-only tables to the
+// earliest level at which both of their attributes are bound.
+int make_plan(amj_ctx_t* ctx, const ProbeUse& probe, const std::vector<TableUse>& uses,
+              Plan* plan, std::vector<uint32_t>* out_attr_ids, std::vector<int>* order);
+
+int build_hash_table(amj_ctx_t* ctx, const amj_relation* rel, uint32_t key_col, int sort_vals,
+                     amj_hash_table** out);
+
+int execute_plan(amj_ctx_t* ctx, Plan& plan, const std::vector<uint32_t>& attr_ids,
+                 const amj_opts_t* opts, amj_result_t* res);
+
+uint32_t floor_log2_u64(uint64_t v);
+double elapsed_ms(hipEvent_t a, hipEvent_t b);
+
+}  // namespace amj

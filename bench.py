@@ -1,0 +1,177 @@
+"""Benchmark: join output tuples/s of the AMHJ 4-way chain join on one MI355X.
+
+Workload (one step): R1(a,b) x R2(b,c) x R3(c,d) x R4(d,e), int32 columns, R1 = 64M rows,
+R2..R4 = 16M rows each, join columns over K = 4M keys: the build-side join column of every
+join is Zipf(s) with a seeded rank->key permutation, the probe-side column uniform; the outer
+columns a, e uniform over [0, 2^31). A step runs the three skews s = 0, 0.5, 1.0 back to back;
+each join builds the three hash tables and probes with ws=1 dro=1 fib=1, materialising all
+~2^32 result tuples (5 int32 columns) in HBM. Inputs are resident in HBM before timing.
+
+Prints one JSON line (see the keys below). --gpus N > 1 exits: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X published HBM3E peak
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def make_inputs(G, scale, s, seed=42):
+    n1 = int((64 << 20) * scale)
+    nb = int((16 << 20) * scale)
+    k = max(1, int((4 << 20) * scale))
+    return G.chain_relations([n1, nb, nb, nb], k, seed=seed, zipf_s=s, payload_range=1 << 31)
+
+
+def algorithmic_bytes(rels, count, n_attrs=5):
+    """Compulsory HBM bytes of one chain join: every input column read once, every build
+    table (key + value + bucket pointers) read once, every result tuple written once."""
+    b = 0
+    for i, (cols, _) in enumerate(rels):
+        n = len(cols[0])
+        b += 4 * len(cols) * n
+        if i > 0:
+            buckets = 1 << (n.bit_length() - 1)
+            b += 8 * n + 4 * (buckets + 1)
+    return b + 4 * n_attrs * count + 16
+
+
+def cpu_baseline(rels, sample_rows, threads):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ffi as O
+    sample = [([c[:sample_rows] for c in rels[0][0]], rels[0][1])] + list(rels[1:])
+    t0 = time.perf_counter()
+    cnt, cs, _, pms, qms = O.mhj(sample, threads=threads)
+    wall = time.perf_counter() - t0
+    return cnt, (pms + qms) / 1e3, wall
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--scale", type=float, default=1.0,
+                    help="fraction of the full workload (1.0 = the configuration above)")
+    ap.add_argument("--skews", default="0,0.5,1.0")
+    ap.add_argument("--ws", type=int, default=1)
+    ap.add_argument("--dro", type=int, default=1)
+    ap.add_argument("--fib", type=int, default=1)
+    ap.add_argument("--cpu-sample-rows", type=int, default=1 << 21)
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from accmultiwayjoins_amd import Context
+    from accmultiwayjoins_amd import generators as G
+
+    skews = [float(x) for x in args.skews.split(",")]
+    t0 = time.perf_counter()
+    inputs = {s: make_inputs(G, args.scale, s) for s in skews}
+    log(f"[bench] generated inputs in {time.perf_counter() - t0:.1f} s")
+
+    ctx = Context(0)
+    dev_name, hbm, cus = ctx.device_info()
+    log(f"[bench] device {dev_name}, {hbm / 2**30:.0f} GiB, {cus} CUs")
+    resident = {}
+    for s in skews:
+        R = [ctx.relation(cols, ids) for cols, ids in inputs[s]]
+        resident[s] = R
+    opts = dict(ws=args.ws, dro=args.dro, fib=args.fib, materialize=True)
+
+    def step(record):
+        tot = 0
+        for s in skews:
+            R = resident[s]
+            H = [ctx.hash_table(r) for r in R[1:]]
+            r = ctx.execute(R[0], H, **opts)
+            tot += r.count
+            if record is not None:
+                record.append((s, r.count, r.checksum, r.build_ms, r.probe_ms,
+                               r.probe_kernel_ms, r.path, 0))
+            ctx.free(r)
+            for h in H:
+                h.close()
+        return tot
+
+    for w in range(args.warmup):
+        t = time.perf_counter()
+        step(None)
+        log(f"[bench] warmup {w} done in {time.perf_counter() - t:.2f} s")
+    N = __import__("accmultiwayjoins_amd._native", fromlist=["lib"])
+    N.lib().amj_sync(ctx.h)
+    rec = []
+    t_start = time.perf_counter()
+    total = 0
+    for k in range(args.steps):
+        total += step(rec)
+        log(f"[bench] step {k} done")
+    N.lib().amj_sync(ctx.h)
+    elapsed = time.perf_counter() - t_start
+    value = total / elapsed
+
+    # roofline of the dominant kernel (result materialisation), from HIP events on its stream
+    last = rec[-len(skews):]
+    k_ms = sum(x[5] for x in last)
+    alg = sum(algorithmic_bytes(inputs[s], cnt) for (s, cnt, *_r) in last)
+    achieved = alg / (k_ms / 1e3) / 1e9 if k_ms > 0 else 0.0
+    per_skew = {str(s): {"count": cnt, "checksum": "%#018x" % cs, "probe_ms": round(pms, 3),
+                         "write_kernel_ms": round(kms, 3), "path": path}
+                for (s, cnt, cs, bms, pms, kms, path, _x) in last}
+
+    cpu = None
+    if not args.no_cpu:
+        s0 = skews[-1]
+        sample_rows = min(args.cpu_sample_rows, len(inputs[s0][0][0][0]))
+        cnt, secs, wall = cpu_baseline(inputs[s0], sample_rows, args.cpu_threads)
+        cpu = {"value": cnt / secs if secs > 0 else 0.0, "unit": "tuples/s",
+               "cores": args.cpu_threads, "kind": "port",
+               "sample": f"oracle MHJ-CPU restatement (partition + probe, count+checksum, no "
+                         f"materialisation) on the s={s0} inputs with R1 cut to its first "
+                         f"{sample_rows} rows; {cnt} results in {secs:.2f} s"}
+        log(f"[bench] cpu baseline: {cnt} tuples in {secs:.2f} s ({wall:.2f} s wall)")
+
+    out = {
+        "metric": "join output tuples/s (4-way chain)",
+        "value": value,
+        "unit": "tuples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic",
+        "config": {"workload": "AMHJ NORMAL 4-way chain, R1 64M + R2..R4 16M int32x2 rows, "
+                               "Zipf s in {0,0.5,1.0} over 4M keys, ws=1 dro=1 fib=1, "
+                               "build + probe + materialise",
+                   "scale": args.scale, "skews": skews, "ws": args.ws, "dro": args.dro,
+                   "fib": args.fib, "per_skew": per_skew},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

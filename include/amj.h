@@ -1,0 +1,145 @@
+/*
+ * amj.h -- C ABI of libamj, the MI355X-native multi-way join library (AMHJ + ALFTJ).
+ *
+ * The reference exposes its operators as templated C++ classes compiled into each CLI.
+ * Each entry point below replaces one of them with plain pointers, sizes and status codes:
+ *
+ *   amj_relation_create      Relation<D,C>::init + read_rel_cols_mmap
+ *                            (MHJ-GPU/Relation.cuh:9-48, common-utils/file_io.h:90-121)
+ *   amj_hash_table_build     QueryProcessor::build_hash_multiway / build_hash_single
+ *                            (MHJ-GPU/QueryProcessor.cuh:76-110, :112-169) and
+ *                            RadixPartitioner::splitKI (MHJ-GPU/Indexing/radix_partitioning.cuh:576-579)
+ *   amj_execute              OOC<D,C,single_join_key,AlgoType>::execute
+ *                            (MHJ-GPU/ooc.cuh:77-165), dispatching like exec_AMHJ.cu:163-198
+ *   amj_chain_join           exec-AMHJ NORMAL (MHJ-GPU/exec_AMHJ.cu:70-89, :101-116, :163-203)
+ *   amj_pattern_join         exec-AMHJ TRI / FOUR (MHJ-GPU/exec_AMHJ.cu:32-69, :118-145)
+ *   amj_trie_build           constructSortedTrie<D,C,2> (LFTJ-GPU/multi_partitioning.cuh:129-293)
+ *   alftj_run                OOCWrapper::execute_with_{DFS,BFS} -> LFTJ_Base::evaluate
+ *                            (LFTJ-GPU/ooc_wrapper.cuh:404-427, LFTJ-GPU/LFTJ_base.h:20)
+ *
+ * Conventions: every function returns AMJ_OK (0) or a negative amj_status instead of the
+ * reference's exit(); amj_last_error() returns the message. The library owns all device
+ * memory it allocates; callers own their inputs. A context owns one HIP stream and must not
+ * be used from two host threads at once.
+ */
+#ifndef AMJ_H_
+#define AMJ_H_
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AMJ_MAX_BUILD_TABLES 5   /* MHJ-GPU/conf.h:18 MAX_NUM_BUILDTABLES */
+#define AMJ_MAX_ATTRS 6          /* MHJ-GPU/conf.h:20 MAX_NUM_RES_ATTRS */
+#define AMJ_MAX_PROBE_COLS 2     /* MHJ-GPU/conf.h:19 MAX_NUM_ATTRS_IN_BUILD_TABLE */
+
+typedef enum {
+  AMJ_OK = 0,
+  AMJ_ERR_ARG = -1,        /* invalid argument / unsupported query shape */
+  AMJ_ERR_HIP = -2,        /* HIP runtime failure */
+  AMJ_ERR_NOMEM = -3,      /* device memory or HBM budget exhausted */
+  AMJ_ERR_NODEV = -4,      /* no usable GPU */
+  AMJ_ERR_STATE = -5       /* object used in the wrong state */
+} amj_status;
+
+typedef enum { AMJ_QUERY_NORMAL = 0, AMJ_QUERY_TRI = 1, AMJ_QUERY_FOUR = 2 } amj_query;
+typedef enum { ALFTJ_BFS = 0, ALFTJ_DFS = 1 } alftj_algo;   /* exec_LFTJ.cu:256-260 */
+
+typedef struct amj_ctx amj_ctx_t;
+typedef struct amj_relation amj_relation_t;
+typedef struct amj_hash_table amj_hash_table_t;
+typedef struct amj_trie amj_trie_t;
+
+/* A relation in the reference's column layout (Relation<D,C>, MHJ-GPU/Relation.cuh:9-48):
+ * data[attr][row] int32, plus the attribute id of each column (attr_list). */
+typedef struct {
+  uint32_t num_cols;
+  uint64_t rows;
+  const int32_t* const* cols;   /* num_cols column pointers */
+  const uint32_t* attr_ids;     /* num_cols attribute ids (may be NULL for TRI/FOUR edges) */
+  int on_device;                /* 1: cols are device pointers that stay valid; 0: host */
+} amj_rel_t;
+
+/* Query options: the exec-AMHJ switches (exec_AMHJ.cu:216-245) plus this library's knobs. */
+typedef struct {
+  int ooc;                      /* stream the probe relation in slices (ooc.cuh:166-306) */
+  int ws;                       /* work sharing between waves */
+  int dro;                      /* direct result output (single pass, no count pass) */
+  int fib;                      /* frequent-item buffer (subtree counts of join values) */
+  int materialize;              /* 1: write result columns; 0: count + checksum only */
+  uint64_t hbm_budget_bytes;    /* 0: use what the device has free */
+  uint64_t ooc_slice_rows;      /* 0: derive from the budget */
+  uint64_t out_chunk_rows;      /* dro: rows per output chunk (0: default) */
+  uint64_t dro_capacity_rows;   /* dro: initial output allocation (0: sampled estimate) */
+  uint32_t ws_split_threshold;  /* ws: split runs longer than this (0: default) */
+  uint32_t reserved[7];
+} amj_opts_t;
+
+/* Result of one join. Columns are device memory owned by the library until
+ * amj_result_free(); column a holds attribute a (renumbered in first-occurrence order). */
+typedef struct {
+  uint64_t count;               /* number of result tuples */
+  uint64_t checksum;            /* sum over tuples of the chained SplitMix64 hash, mod 2^64 */
+  uint32_t num_cols;
+  int32_t* cols[AMJ_MAX_ATTRS];
+  uint64_t rows_written;        /* rows materialised (== count when materialize=1) */
+  uint32_t attr_ids[AMJ_MAX_ATTRS];  /* original attribute id of each output column */
+  double build_ms;              /* hash build / trie build time (device, HIP events) */
+  double probe_ms;              /* probe + output time (device, HIP events) */
+  double probe_kernel_ms;       /* time of the dominant probe kernel only */
+  uint64_t probe_kernel_launches;
+  uint32_t path;                /* which engine ran (AMJ_PATH_*) */
+  uint32_t reserved[5];
+} amj_result_t;
+
+#define AMJ_PATH_PLAIN 1        /* count pass + scan + write pass */
+#define AMJ_PATH_FIB 2          /* subtree-count buffer + output-window writer */
+#define AMJ_PATH_WS 3
+#define AMJ_PATH_DRO 4
+#define AMJ_PATH_LFTJ_DFS 5
+#define AMJ_PATH_LFTJ_BFS 6
+
+/* ---- context ------------------------------------------------------------------------ */
+int amj_ctx_create(int device, amj_ctx_t** out);
+void amj_ctx_destroy(amj_ctx_t* ctx);
+const char* amj_last_error(const amj_ctx_t* ctx);
+int amj_sync(amj_ctx_t* ctx);
+int amj_device_info(amj_ctx_t* ctx, char* name, size_t name_len, uint64_t* hbm_bytes,
+                    int* cu_count);
+void amj_default_opts(amj_opts_t* opts);
+const char* amj_version(void);
+
+/* ---- relations and indexes ---------------------------------------------------------- */
+int amj_relation_create(amj_ctx_t* ctx, const amj_rel_t* rel, amj_relation_t** out);
+void amj_relation_destroy(amj_ctx_t* ctx, amj_relation_t* rel);
+/* Build the bucketed hash table of `rel` on column key_col, with B = floorPowerOf2(rows)
+ * buckets (QueryProcessor.cuh:86). sort_vals=1 also orders the other column inside each
+ * key run (needed when that column is compared rather than bound). */
+int amj_hash_table_build(amj_ctx_t* ctx, const amj_relation_t* rel, uint32_t key_col,
+                         int sort_vals, amj_hash_table_t** out);
+void amj_hash_table_destroy(amj_ctx_t* ctx, amj_hash_table_t* ht);
+
+/* ---- AMHJ ----------------------------------------------------------------------------- */
+/* OOC::execute: rels[0] is the probe relation, ht[i] is built on rels[i+1] (its attribute
+ * list is `attr_ids` of that relation, hash column first). Output attributes follow the
+ * reference's first-occurrence rule (exec_AMHJ.cu:76-88). */
+int amj_execute(amj_ctx_t* ctx, const amj_relation_t* probe, amj_hash_table_t* const* hts,
+                uint32_t n_hts, const amj_opts_t* opts, amj_result_t* res);
+/* exec-AMHJ NORMAL in one call: relations in chain order, rels[0] = probe. */
+int amj_chain_join(amj_ctx_t* ctx, const amj_rel_t* rels, uint32_t n, const amj_opts_t* opts,
+                   amj_result_t* res);
+/* exec-AMHJ TRI / FOUR over one two-column edge relation (exec_AMHJ.cu:32-69). */
+int amj_pattern_join(amj_ctx_t* ctx, const amj_rel_t* edges, int query, const amj_opts_t* opts,
+                     amj_result_t* res);
+int amj_result_free(amj_ctx_t* ctx, amj_result_t* res);
+/* Copy rows [row0, row0+n) of result column `col` to host memory. */
+int amj_result_copy_col(amj_ctx_t* ctx, const amj_result_t* res, uint32_t col, uint64_t row0,
+                        uint64_t n, int32_t* host_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AMJ_H_ */

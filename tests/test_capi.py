@@ -1,0 +1,109 @@
+"""CPU-side checks of the C ABI and the host generators (no GPU needed)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+from accmultiwayjoins_amd import _native as N
+from accmultiwayjoins_amd import generators as G
+
+ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), ".."))
+
+
+def declared_functions(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b((?:amj|alftj)_[a-z0-9_]+)\s*\(", text)))
+
+
+@pytest.mark.parametrize("header,path", [("amj.h", N.LIB_PATH), ("amj_gen.h", N.GEN_PATH)])
+def test_library_exports_every_declared_symbol(header, path):
+    assert os.path.exists(path), f"{path} not built"
+    lib = C.CDLL(path)
+    names = declared_functions(header)
+    assert names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_python_binding_covers_header():
+    bound = {n for n, _, _ in N.SIGNATURES}
+    assert set(declared_functions("amj.h")) <= bound
+
+
+def test_struct_layout_matches_header(tmp_path):
+    # the ctypes mirrors must have the offsets the C compiler gives the structs of amj.h
+    import subprocess
+    structs = {"amj_rel_t": N.AmjRel, "amj_opts_t": N.AmjOpts, "amj_result_t": N.AmjResult}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "amj.h"', "int main(void){"]
+    for cname, py in structs.items():
+        lines.append('printf("%s sizeof %%zu\\n", sizeof(%s));' % (cname, cname))
+        for f, _ in py._fields_:
+            lines.append('printf("%s %s %%zu\\n", offsetof(%s, %s));' % (cname, f, cname, f))
+    lines.append("return 0;}")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.check_call(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)])
+    got = {}
+    for line in subprocess.check_output([str(exe)], text=True).splitlines():
+        s, f, v = line.split()
+        got[(s, f)] = int(v)
+    for cname, py in structs.items():
+        assert got[(cname, "sizeof")] == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
+
+
+def test_no_gpu_context_fails_cleanly():
+    import subprocess
+    import sys
+    # Without a GPU the context constructor must return AMJ_ERR_NODEV, not crash.
+    code = ("import ctypes as C,sys; sys.path.insert(0,%r); from accmultiwayjoins_amd import "
+            "_native as N; h=C.c_void_p(); print(N.lib().amj_ctx_create(0, C.byref(h)))" % ROOT)
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True,
+                         timeout=120)
+    assert out.returncode == 0, out.stderr
+    rc = int(out.stdout.strip().splitlines()[-1])
+    assert rc in (0, -4)
+
+
+def test_generators_deterministic_and_thread_independent():
+    a = G.uniform(100000, -7, 1000, seed=42, stream=3)
+    b = G.uniform(100000, -7, 1000, seed=42, stream=3)
+    assert np.array_equal(a, b)
+    assert a.min() >= -7 and a.max() < 1000
+    c = G.uniform(100000, -7, 1000, seed=42, stream=4)
+    assert not np.array_equal(a, c)
+    # prefix property: row i depends only on (seed, stream, i)
+    assert np.array_equal(G.uniform(1000, -7, 1000, seed=42, stream=3), a[:1000])
+
+
+def test_zipf_distribution():
+    k = 1000
+    z0 = G.zipf(200000, k, 0.0, seed=1, stream=0)
+    z1 = G.zipf(200000, k, 1.0, seed=1, stream=0)
+    assert z0.min() >= 0 and z0.max() < k and z1.min() >= 0 and z1.max() < k
+    h1 = np.sort(np.bincount(z1, minlength=k))[::-1]
+    h0 = np.sort(np.bincount(z0, minlength=k))[::-1]
+    # s = 1: frequency of rank r ~ 1/(r H_k); top key ~ 200000 / H_1000 ~ 26700
+    harm = sum(1.0 / r for r in range(1, k + 1))
+    assert abs(h1[0] - 200000 / harm) < 0.05 * 200000 / harm
+    assert h0[0] < 400  # uniform: ~200 per key
+    # rank->key map is a permutation: the hottest key is not simply key 0
+    assert np.argmax(np.bincount(z1, minlength=k)) == np.argmax(np.bincount(
+        G.zipf(50000, k, 1.0, seed=1, stream=0), minlength=k))
+
+
+def test_rmat_orient_properties():
+    s, d = G.rmat_oriented(10, 8, seed=9)
+    assert len(s) > 0
+    assert np.all(s != d)
+    pairs = s.astype(np.int64) << 32 | d.astype(np.int64)
+    assert np.all(np.diff(pairs) > 0)  # sorted by (src, dst), no duplicates
+    und = set(zip(np.minimum(s, d).tolist(), np.maximum(s, d).tolist()))
+    assert len(und) == len(s)  # each undirected edge once
+    deg = np.bincount(np.concatenate([s, d]), minlength=1 << 10)
+    assert np.all((deg[s] < deg[d]) | ((deg[s] == deg[d]) & (s < d)))

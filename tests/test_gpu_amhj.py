@@ -1,0 +1,144 @@
+"""GPU parity of the AMHJ multi-way hash join (libamj through its C ABI) against the CPU
+oracle, the reference MHJ-CPU golden counts and brute force. Integer results are bit-exact:
+count, order-independent checksum and, for small inputs, the sorted tuples."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle_ffi as O
+from accmultiwayjoins_amd import AmjError
+from accmultiwayjoins_amd import generators as G
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "mhj_cpu_counts.json")))
+
+PATHS = [dict(), dict(fib=1)]
+PATH_IDS = ["plain", "fib"]
+
+
+def _rels(rec):
+    return G.chain_relations(rec["sizes"], rec["key_range"], seed=rec["seed"],
+                             zipf_s=rec["zipf_s"], payload_range=rec["payload_range"])
+
+
+@pytest.mark.parametrize("opts", PATHS, ids=PATH_IDS)
+@pytest.mark.parametrize("rec", GOLDEN["datasets"], ids=lambda r: r["name"])
+def test_chain_matches_reference_counts_and_oracle_checksum(ctx, rec, opts):
+    rels = _rels(rec)
+    want_cnt, want_cs, _, _, _ = O.mhj(rels)
+    assert want_cnt == rec["count"]
+    r = ctx.chain_join(rels, materialize=True, fetch=False, **opts)
+    try:
+        assert r.count == rec["count"]
+        assert r.checksum == want_cs
+        assert r.rows_written == r.count
+    finally:
+        ctx.free(r)
+    r2 = ctx.chain_join(rels, materialize=False, **opts)
+    assert (r2.count, r2.checksum) == (want_cnt, want_cs)
+    assert r2.rows_written == 0
+
+
+def _random_chain(rng, n_rel, max_rows, keys, lo=0, empty=None):
+    rels = []
+    for i in range(n_rel):
+        n = 0 if empty == i else int(rng.integers(1, max_rows))
+        cols = [rng.integers(lo, lo + keys, size=n).astype(np.int32) for _ in range(2)]
+        rels.append((cols, [i, i + 1]))
+    return rels
+
+
+@pytest.mark.parametrize("opts", PATHS, ids=PATH_IDS)
+@pytest.mark.parametrize("case", range(16))
+def test_chain_small_tuples_equal_brute_force(ctx, case, opts):
+    rng = np.random.default_rng(1000 + case)
+    n_rel = 2 + case % 5  # 2..6 relations
+    rels = _random_chain(rng, n_rel, 300, keys=int(rng.integers(2, 40)),
+                         lo=-20 if case % 4 == 1 else 0, empty=(1 if case == 7 else None))
+    want = O.brute_join(rels)
+    r = ctx.chain_join(rels, fetch=True, **opts)
+    assert r.count == len(want)
+    got = O.sort_tuples(r.tuples()) if r.count else np.zeros((0, n_rel + 1), np.int32)
+    assert [tuple(int(x) for x in t) for t in got] == want
+    assert r.checksum == O.checksum(want)
+
+
+def test_fib_and_plain_write_identical_arrays(ctx):
+    # both paths emit results in the same depth-first order over probe rows
+    rels = G.chain_relations([20000, 5000, 7000, 3000], 300, seed=3, zipf_s=1.0)
+    a = ctx.chain_join(rels, fetch=True)
+    b = ctx.chain_join(rels, fetch=True, fib=1, out_chunk_rows=1000)
+    assert a.path == "plain" and b.path == "fib"
+    assert a.count == b.count > 0
+    for ca, cb in zip(a.cols, b.cols):
+        assert np.array_equal(ca, cb)
+
+
+def test_heavy_skew_single_key(ctx):
+    # one build key holding every row: runs of 200k rows, results 50 * 200k * 3
+    n = 200_000
+    probe = ([np.arange(50, dtype=np.int32), np.full(50, 7, np.int32)], [0, 1])
+    b1 = ([np.full(n, 7, np.int32), np.arange(n, dtype=np.int32) % 3], [1, 2])
+    b2 = ([np.arange(3, dtype=np.int32), np.array([10, 11, 12], np.int32)], [2, 3])
+    rels = [probe, b1, b2]
+    want_cnt, want_cs, _, _, _ = O.mhj(rels)
+    assert want_cnt == 50 * n
+    for opts in PATHS:
+        r = ctx.chain_join(rels, materialize=True, **opts)
+        assert (r.count, r.checksum) == (want_cnt, want_cs)
+        ctx.free(r)
+
+
+def test_kat_triangle_through_chain_join(ctx):
+    from test_oracle import KAT_TRIANGLE, kat_relations
+    r = ctx.chain_join(kat_relations(), fetch=True)
+    assert [tuple(int(x) for x in t) for t in O.sort_tuples(r.tuples())] == KAT_TRIANGLE
+
+
+def test_star_and_cycle_general_compare(ctx):
+    rng = np.random.default_rng(5)
+    a = [rng.integers(0, 9, 500).astype(np.int32) for _ in range(2)]
+    b = [rng.integers(0, 9, 400).astype(np.int32) for _ in range(2)]
+    c = [rng.integers(0, 9, 300).astype(np.int32) for _ in range(2)]
+    for rels in ([(a, [0, 1]), (b, [0, 2]), (c, [0, 3])],
+                 [(a, [0, 1]), (b, [1, 2]), (c, [0, 2])]):
+        want = O.brute_join(rels)
+        r = ctx.chain_join(rels, fetch=True)
+        assert r.count == len(want)
+        assert [tuple(int(x) for x in t) for t in O.sort_tuples(r.tuples())] == want
+
+
+@pytest.mark.parametrize("scale", [8, 11])
+def test_pattern_tri_four_match_graph_oracle(ctx, scale):
+    s, d = G.rmat_oriented(scale, 16, seed=42)
+    n3, c3 = O.triangles(s, d)
+    n4, c4 = O.four_cliques(s, d)
+    r3 = ctx.pattern_join(s, d, "TRI", materialize=False)
+    assert (r3.count, r3.checksum) == (n3, c3)
+    r4 = ctx.pattern_join(s, d, "FOUR", materialize=False)
+    assert (r4.count, r4.checksum) == (n4, c4)
+    r3m = ctx.pattern_join(s, d, "TRI", fetch=True)
+    assert r3m.count == n3
+    assert O.checksum(r3m.tuples()) == c3
+
+
+def test_resident_relations_and_tables_reused(ctx):
+    rels = G.chain_relations([5000, 3000, 3000], 500, seed=9)
+    want_cnt, want_cs, _, _, _ = O.mhj(rels)
+    R = [ctx.relation(c, ids) for c, ids in rels]
+    H = [ctx.hash_table(r) for r in R[1:]]
+    for _ in range(3):
+        r = ctx.execute(R[0], H, materialize=False)
+        assert (r.count, r.checksum) == (want_cnt, want_cs)
+
+
+def test_errors_are_reported_not_fatal(ctx):
+    # hash attribute of the second build table is never bound before it
+    rels = [([np.arange(4, dtype=np.int32)] * 2, [0, 1]),
+            ([np.arange(4, dtype=np.int32)] * 2, [5, 6])]
+    with pytest.raises(AmjError) as e:
+        ctx.chain_join(rels)
+    assert "not bound" in str(e.value)
