@@ -55,7 +55,7 @@ def _random_chain(rng, n_rel, max_rows, keys, lo=0, empty=None):
 @pytest.mark.parametrize("case", range(16))
 def test_chain_small_tuples_equal_brute_force(ctx, case, opts):
     rng = np.random.default_rng(1000 + case)
-    n_rel = 2 + case % 5  # 2..6 relations
+    n_rel = 2 + case % 4  # 2..5 relations: at most 6 result attributes (MHJ-GPU/conf.h:20)
     rels = _random_chain(rng, n_rel, 300, keys=int(rng.integers(2, 40)),
                          lo=-20 if case % 4 == 1 else 0, empty=(1 if case == 7 else None))
     want = O.brute_join(rels)
@@ -133,6 +133,14 @@ def test_resident_relations_and_tables_reused(ctx):
     for _ in range(3):
         r = ctx.execute(R[0], H, materialize=False)
         assert (r.count, r.checksum) == (want_cnt, want_cs)
+
+
+def test_too_many_attributes_rejected(ctx):
+    rng = np.random.default_rng(1)
+    rels = _random_chain(rng, 6, 50, keys=5)  # 7 attributes > AMJ_MAX_ATTRS
+    with pytest.raises(AmjError) as e:
+        ctx.chain_join(rels)
+    assert "too many attributes" in str(e.value)
 
 
 def test_errors_are_reported_not_fatal(ctx):
