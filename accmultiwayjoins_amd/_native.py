@@ -41,6 +41,7 @@ class AmjOpts(C.Structure):
         ("dro", C.c_int),
         ("fib", C.c_int),
         ("materialize", C.c_int),
+        ("checksum", C.c_int),
         ("hbm_budget_bytes", C.c_uint64),
         ("ooc_slice_rows", C.c_uint64),
         ("out_chunk_rows", C.c_uint64),
@@ -82,6 +83,9 @@ SIGNATURES = [
     ("amj_hash_table_build", C.c_int,
      [C.c_void_p, C.c_void_p, C.c_uint32, C.c_int, C.POINTER(C.c_void_p)]),
     ("amj_hash_table_destroy", None, [C.c_void_p, C.c_void_p]),
+    ("amj_hash_table_info", C.c_int,
+     [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+      C.POINTER(C.c_int), C.POINTER(C.c_double)]),
     ("amj_execute", C.c_int,
      [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(AmjOpts),
       C.POINTER(AmjResult)]),

@@ -50,12 +50,14 @@ class JoinResult:
         return np.stack(self.cols, axis=1)
 
 
-def _opts(ooc=0, ws=0, dro=0, fib=0, materialize=True, hbm_budget_bytes=0, ooc_slice_rows=0,
-          out_chunk_rows=0, dro_capacity_rows=0, ws_split_threshold=0) -> N.AmjOpts:
+def _opts(ooc=0, ws=0, dro=0, fib=0, materialize=True, checksum=True, hbm_budget_bytes=0,
+          ooc_slice_rows=0, out_chunk_rows=0, dro_capacity_rows=0,
+          ws_split_threshold=0) -> N.AmjOpts:
     o = N.AmjOpts()
     N.lib().amj_default_opts(C.byref(o))
     o.ooc, o.ws, o.dro, o.fib = int(ooc), int(ws), int(dro), int(fib)
     o.materialize = 1 if materialize else 0
+    o.checksum = 1 if checksum else 0
     o.hbm_budget_bytes = int(hbm_budget_bytes)
     o.ooc_slice_rows = int(ooc_slice_rows)
     o.out_chunk_rows = int(out_chunk_rows)
@@ -113,6 +115,11 @@ class HashTable:
         ctx._check(N.lib().amj_hash_table_build(ctx.h, rel.h, key_col, int(sort_vals),
                                                 C.byref(h)))
         self.ctx, self.h, self.rel = ctx, h, rel
+        rows, buckets, ident, ms = C.c_uint64(), C.c_uint64(), C.c_int(), C.c_double()
+        ctx._check(N.lib().amj_hash_table_info(ctx.h, h, C.byref(rows), C.byref(buckets),
+                                               C.byref(ident), C.byref(ms)))
+        self.rows, self.buckets = rows.value, buckets.value
+        self.identity, self.build_ms = bool(ident.value), ms.value
 
     def close(self):
         if self.h:

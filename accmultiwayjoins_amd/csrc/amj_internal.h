@@ -66,6 +66,14 @@ struct DevTable {
   uint32_t pad;
 };
 
+// Run of table t+1 that a row of table t (or a probe row) leads to, with the weight prefix at
+// its first row (frequent-item path).
+struct RunRef {
+  uint32_t lo;
+  uint32_t len;
+  uint64_t pwlo;
+};
+
 // How a build table's second column relates to the tuple built so far.
 enum ValMode : int32_t { VAL_NONE = 0, VAL_BIND = 1, VAL_CHECK = 2 };
 
@@ -84,6 +92,8 @@ struct Plan {
   int32_t val_attr[kMaxLevels];       // attribute of the second column (-1: none)
   int32_t val_mode[kMaxLevels];       // ValMode
   int32_t pad1;
+  const RunRef* next[kMaxLevels];     // frequent-item path: next[t][r] = run in table t+1
+  const RunRef* probe_next;           // frequent-item path: run in table 0 of each probe row
 };
 
 // ---------------------------------------------------------------------------------------
@@ -215,18 +225,22 @@ struct ProbeOut {
 // Count pass: per-unit result counts (unit = 64 consecutive probe rows).
 hipError_t launch_probe_count(const Plan& plan, uint64_t* unit_counts, hipStream_t s);
 // Write pass: units write at unit_base[u] (exclusive scan of counts); per-block checksum
-// partials go to block_sums.
+// partials go to block_sums. hash=false skips the checksum when materialising.
 hipError_t launch_probe_write(const Plan& plan, const uint64_t* unit_base, const ProbeOut& out,
-                              uint64_t* block_sums, uint64_t* n_blocks_out, hipStream_t s);
+                              bool hash, uint64_t* block_sums, uint64_t* n_blocks_out,
+                              hipStream_t s);
 uint64_t probe_units(const Plan& plan);
 uint64_t probe_write_blocks(const Plan& plan);
 
 // Frequent-item (subtree-count) buffer for linear chains.
-hipError_t launch_fib_weights(const Plan& plan, int level, uint64_t* w, hipStream_t s);
-hipError_t launch_fib_probe_weights(const Plan& plan, uint64_t* w, hipStream_t s);
+hipError_t launch_fib_weights(const Plan& plan, int level, uint64_t* w, RunRef* next,
+                              hipStream_t s);
+hipError_t launch_fib_probe_weights(const Plan& plan, uint64_t* w, RunRef* next, hipStream_t s);
 // Output-window writer: every wave materialises outputs [w*win, (w+1)*win) exactly.
-hipError_t launch_window_write(const Plan& plan, const uint64_t* probe_off, uint64_t total,
-                               uint64_t window, const ProbeOut& out, uint64_t* block_sums,
-                               uint64_t* n_blocks_out, hipStream_t s);
+uint64_t window_blocks(uint64_t total, uint64_t window);
+hipError_t launch_window_write(const Plan& plan, const uint64_t* probe_off, uint64_t* first_row,
+                               uint64_t total, uint64_t window, const ProbeOut& out, bool hash,
+                               uint64_t* block_sums, uint32_t* overflow, uint64_t* n_blocks_out,
+                               hipStream_t s);
 
 }  // namespace amj

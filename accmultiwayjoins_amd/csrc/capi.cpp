@@ -13,6 +13,7 @@ void amj_default_opts(amj_opts_t* o) {
   if (!o) return;
   std::memset(o, 0, sizeof(*o));
   o->materialize = 1;
+  o->checksum = 1;
 }
 
 int amj_ctx_create(int device, amj_ctx_t** out) {
@@ -123,6 +124,16 @@ void amj_hash_table_destroy(amj_ctx_t* ctx, amj_hash_table_t* ht) {
   dfree(ctx, ht->dev.keys, n * 4);
   dfree(ctx, ht->dev.vals, n * 4);
   delete ht;
+}
+
+int amj_hash_table_info(amj_ctx_t* ctx, const amj_hash_table_t* ht, uint64_t* rows,
+                        uint64_t* buckets, int* identity, double* build_ms) {
+  if (!ctx || !ht) return AMJ_ERR_ARG;
+  if (rows) *rows = ht->dev.view.n;
+  if (buckets) *buckets = ht->dev.buckets;
+  if (identity) *identity = (int)ht->dev.view.identity;
+  if (build_ms) *build_ms = ht->build_ms;
+  return AMJ_OK;
 }
 
 static TableUse use_of(const amj_hash_table* ht) {

@@ -180,7 +180,15 @@ __global__ __launch_bounds__(kBlock) void key_diff_kernel(const uint64_t* __rest
     acc |= keys[i] ^ k0;
 #pragma unroll
   for (int d = kWave / 2; d >= 1; d >>= 1) acc |= __shfl_xor(acc, d, kWave);
-  if (lane_id() == 0 && acc) atomicOr((unsigned long long*)diff, (unsigned long long)acc);
+  __shared__ uint64_t part[kWavesPerBlock];
+  if (lane_id() == 0) part[threadIdx.x / kWave] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t v = 0;
+#pragma unroll
+    for (int i = 0; i < kWavesPerBlock; ++i) v |= part[i];
+    if (v) atomicOr((unsigned long long*)diff, (unsigned long long)v);
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void radix_hist_kernel(const uint64_t* __restrict__ keys,
@@ -261,7 +269,7 @@ hipError_t radix_sort_u64(uint64_t* keys, uint64_t* alt, uint64_t n, uint32_t di
 
   hipError_t e = hipMemsetAsync(diff, 0, sizeof(uint64_t), s);
   if (e != hipSuccess) return e;
-  const unsigned grid = (unsigned)std::min<uint64_t>((n + kBlock - 1) / kBlock, 4096);
+  const unsigned grid = (unsigned)std::min<uint64_t>((n + kBlock - 1) / kBlock, 1024);
   hipLaunchKernelGGL(key_diff_kernel, dim3(grid), dim3(kBlock), 0, s, keys, n, diff);
   uint64_t hdiff = 0;
   e = hipMemcpyAsync(&hdiff, diff, sizeof(uint64_t), hipMemcpyDeviceToHost, s);

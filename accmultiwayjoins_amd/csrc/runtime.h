@@ -59,6 +59,9 @@ enum ScratchSlot {
   SLOT_PROBE_OFF,
   SLOT_PW0,  // SLOT_PW0 + t for level t
   SLOT_PW_END = SLOT_PW0 + kMaxLevels,
+  SLOT_NEXT0,  // SLOT_NEXT0 + t: run pointers of table t into table t+1
+  SLOT_NEXT_END = SLOT_NEXT0 + kMaxLevels,
+  SLOT_PROBE_NEXT,
   SLOT_FIB_W,
   SLOT_OOC_A,
   SLOT_OOC_B,

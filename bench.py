@@ -93,7 +93,7 @@ def main():
     for s in skews:
         R = [ctx.relation(cols, ids) for cols, ids in inputs[s]]
         resident[s] = R
-    opts = dict(ws=args.ws, dro=args.dro, fib=args.fib, materialize=True)
+    opts = dict(ws=args.ws, dro=args.dro, fib=args.fib, materialize=True, checksum=False)
 
     def step(record):
         tot = 0
@@ -103,12 +103,31 @@ def main():
             r = ctx.execute(R[0], H, **opts)
             tot += r.count
             if record is not None:
-                record.append((s, r.count, r.checksum, r.build_ms, r.probe_ms,
+                record.append((s, r.count, r.checksum, sum(h.build_ms for h in H), r.probe_ms,
                                r.probe_kernel_ms, r.path, 0))
             ctx.free(r)
             for h in H:
                 h.close()
         return tot
+
+    def verify():
+        """Untimed: the materialising path with its checksum on must agree with the
+        independent count-pass engine (count + checksum, no materialisation)."""
+        out = {}
+        for s in skews:
+            R = resident[s]
+            H = [ctx.hash_table(r) for r in R[1:]]
+            a = ctx.execute(R[0], H, **dict(opts, checksum=True))
+            ctx.free(a)
+            b = ctx.execute(R[0], H, materialize=False)
+            for h in H:
+                h.close()
+            if (a.count, a.checksum) != (b.count, b.checksum):
+                raise SystemExit(f"bench: verification failed at s={s}: "
+                                 f"{a.path} {a.count}/{a.checksum:#x} vs "
+                                 f"{b.path} {b.count}/{b.checksum:#x}")
+            out[s] = (a.count, a.checksum, a.path, b.path)
+        return out
 
     for w in range(args.warmup):
         t = time.perf_counter()
@@ -131,8 +150,12 @@ def main():
     k_ms = sum(x[5] for x in last)
     alg = sum(algorithmic_bytes(inputs[s], cnt) for (s, cnt, *_r) in last)
     achieved = alg / (k_ms / 1e3) / 1e9 if k_ms > 0 else 0.0
-    per_skew = {str(s): {"count": cnt, "checksum": "%#018x" % cs, "probe_ms": round(pms, 3),
-                         "write_kernel_ms": round(kms, 3), "path": path}
+    checked = verify()
+    log("[bench] verification: materialising path == count-pass engine for every skew")
+    per_skew = {str(s): {"count": cnt, "checksum": "%#018x" % checked[s][1],
+                         "build_ms": round(bms, 3), "probe_ms": round(pms, 3),
+                         "write_kernel_ms": round(kms, 3), "path": path,
+                         "verified_against": checked[s][3]}
                 for (s, cnt, cs, bms, pms, kms, path, _x) in last}
 
     cpu = None

@@ -70,6 +70,8 @@ typedef struct {
   int dro;                      /* direct result output (single pass, no count pass) */
   int fib;                      /* frequent-item buffer (subtree counts of join values) */
   int materialize;              /* 1: write result columns; 0: count + checksum only */
+  int checksum;                 /* 1: compute the result checksum (always on when
+                                   materialize = 0) */
   uint64_t hbm_budget_bytes;    /* 0: use what the device has free */
   uint64_t ooc_slice_rows;      /* 0: derive from the budget */
   uint64_t out_chunk_rows;      /* dro: rows per output chunk (0: default) */
@@ -121,6 +123,9 @@ void amj_relation_destroy(amj_ctx_t* ctx, amj_relation_t* rel);
 int amj_hash_table_build(amj_ctx_t* ctx, const amj_relation_t* rel, uint32_t key_col,
                          int sort_vals, amj_hash_table_t** out);
 void amj_hash_table_destroy(amj_ctx_t* ctx, amj_hash_table_t* ht);
+/* Size and build statistics of a hash table (HashTable::get_ht_size, Relation.cuh:235-247). */
+int amj_hash_table_info(amj_ctx_t* ctx, const amj_hash_table_t* ht, uint64_t* rows,
+                        uint64_t* buckets, int* identity, double* build_ms);
 
 /* ---- AMHJ ----------------------------------------------------------------------------- */
 /* OOC::execute: rels[0] is the probe relation, ht[i] is built on rels[i+1] (its attribute
