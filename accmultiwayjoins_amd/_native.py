@@ -47,7 +47,8 @@ class AmjOpts(C.Structure):
         ("out_chunk_rows", C.c_uint64),
         ("dro_capacity_rows", C.c_uint64),
         ("ws_split_threshold", C.c_uint32),
-        ("reserved", C.c_uint32 * 7),
+        ("fib_suffix", C.c_int),
+        ("reserved", C.c_uint32 * 6),
     ]
 
 

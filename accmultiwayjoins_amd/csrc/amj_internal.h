@@ -66,12 +66,14 @@ struct DevTable {
   uint32_t pad;
 };
 
-// Run of table t+1 that a row of table t (or a probe row) leads to, with the weight prefix at
-// its first row (frequent-item path).
+// One 16-byte record per row of table t (or probe row) for the frequent-item path: the value the
+// row binds and the run [lo, lo+len) of table t+1 that the value leads to. Packing them keeps the
+// data a child needs in one cache line per run of consecutive rows.
 struct RunRef {
+  int32_t val;
   uint32_t lo;
   uint32_t len;
-  uint64_t pwlo;
+  uint32_t pad;
 };
 
 // How a build table's second column relates to the tuple built so far.
@@ -92,8 +94,14 @@ struct Plan {
   int32_t val_attr[kMaxLevels];       // attribute of the second column (-1: none)
   int32_t val_mode[kMaxLevels];       // ValMode
   int32_t pad1;
-  const RunRef* next[kMaxLevels];     // frequent-item path: next[t][r] = run in table t+1
-  const RunRef* probe_next;           // frequent-item path: run in table 0 of each probe row
+  // Output-window writer (frequent-item path) only:
+  const RunRef* next[kMaxLevels];     // next[t][r]: binding of row r of table t and its run
+                                      // one level down (table t+1, or the suffix table)
+  const RunRef* probe_next;           // run of each probe row in table 0 (or the suffix table)
+  const int32_t* sfx_base;            // suffix table: column k at sfx_base + k * sfx_stride
+  uint64_t sfx_stride;
+  int32_t sfx_cols;                   // columns of the suffix table (0..kMaxAttrs)
+  int32_t pad2;
 };
 
 // ---------------------------------------------------------------------------------------
@@ -238,6 +246,8 @@ hipError_t launch_fib_weights(const Plan& plan, int level, uint64_t* w, RunRef* 
 hipError_t launch_fib_probe_weights(const Plan& plan, uint64_t* w, RunRef* next, hipStream_t s);
 // Output-window writer: every wave materialises outputs [w*win, (w+1)*win) exactly.
 uint64_t window_blocks(uint64_t total, uint64_t window);
+bool window_plan_supported(const Plan& plan);
+hipError_t launch_records_to_suffix(RunRef* rec, uint64_t n, const uint64_t* pw, hipStream_t s);
 hipError_t launch_window_write(const Plan& plan, const uint64_t* probe_off, uint64_t* first_row,
                                uint64_t total, uint64_t window, const ProbeOut& out, bool hash,
                                uint64_t* block_sums, uint32_t* overflow, uint64_t* n_blocks_out,

@@ -235,11 +235,11 @@ __device__ __forceinline__ uint64_t pw_at(const Plan& P, int t, uint32_t x) {
 __device__ __forceinline__ uint64_t run_of(const Plan& P, int t, int32_t key, RunRef* out) {
   uint32_t lo, len;
   lookup_run(P.tab[t], key, false, 0, true, lo, len);
-  const uint64_t pwlo = pw_at(P, t, lo);
+  out->val = key;
   out->lo = lo;
   out->len = len;
-  out->pwlo = pwlo;
-  return pw_at(P, t, lo + len) - pwlo;
+  out->pad = 0;
+  return pw_at(P, t, lo + len) - pw_at(P, t, lo);
 }
 
 // rows of table t: w[r] = results below row r (w[n] = 0), next[r] = run in table t+1

@@ -7,8 +7,6 @@
 
 namespace amj {
 
-constexpr uint64_t kDefaultWindow = 65536;  // results per wave in the output-window writer
-
 void set_error(amj_ctx_t* ctx, int code, const std::string& msg) {
   (void)code;
   if (ctx) ctx->last_error = msg;
@@ -245,7 +243,7 @@ int make_plan(amj_ctx_t* ctx, const ProbeUse& probe, const std::vector<TableUse>
   return AMJ_OK;
 }
 
-static bool fib_eligible(const Plan& P) {
+bool fib_eligible(const Plan& P) {
   for (int t = 0; t < P.n_levels; ++t) {
     if (P.val_mode[t] == VAL_CHECK) return false;
     if (t + 1 < P.n_levels) {
@@ -257,7 +255,7 @@ static bool fib_eligible(const Plan& P) {
 }
 
 // Output columns: reuse a parked allocation when it is large enough.
-static int alloc_output(amj_ctx_t* ctx, uint64_t rows, int n_attrs, amj_result_t* res) {
+int alloc_output(amj_ctx_t* ctx, uint64_t rows, int n_attrs, amj_result_t* res) {
   const size_t bytes = std::max<uint64_t>(rows, 1) * sizeof(int32_t) * (size_t)n_attrs;
   void* base = nullptr;
   size_t got = 0;
@@ -283,7 +281,7 @@ static int alloc_output(amj_ctx_t* ctx, uint64_t rows, int n_attrs, amj_result_t
   return AMJ_OK;
 }
 
-static int finish_checksum(amj_ctx_t* ctx, const uint64_t* block_sums, uint64_t nb,
+int finish_checksum(amj_ctx_t* ctx, const uint64_t* block_sums, uint64_t nb,
                            uint64_t* out) {
   void *sum = nullptr, *tmp = nullptr;
   AMJ_TRY(scratch(ctx, SLOT_SUM, 64, &sum));
@@ -294,7 +292,7 @@ static int finish_checksum(amj_ctx_t* ctx, const uint64_t* block_sums, uint64_t 
   return AMJ_OK;
 }
 
-static uint64_t budget_left(amj_ctx_t* ctx, const amj_opts_t* o) {
+uint64_t budget_left(amj_ctx_t* ctx, const amj_opts_t* o) {
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
   uint64_t left = free_b;
@@ -354,91 +352,6 @@ static int run_plain(amj_ctx_t* ctx, Plan& P, const amj_opts_t* o, amj_result_t*
   return AMJ_OK;
 }
 
-// Frequent-item buffer (all join values) + output-window writer. Returns AMJ_ERR_STATE when
-// a frontier level would exceed 2^32 children (then the caller uses the count/write path).
-static int run_fib(amj_ctx_t* ctx, Plan& P, const amj_opts_t* o, amj_result_t* res) {
-  hipStream_t s = ctx->stream;
-  const int L = P.n_levels;
-  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[2], s));
-  void* w = nullptr;
-  void* tmp = nullptr;
-  for (int t = L - 2; t >= 0; --t) {  // bottom-up: weights of table t need PW of table t+1
-    const uint64_t n = P.tab[t].n;
-    void *pw = nullptr, *nx = nullptr;
-    AMJ_TRY(scratch(ctx, SLOT_PW0 + t, (n + 1) * 8, &pw));
-    AMJ_TRY(scratch(ctx, SLOT_NEXT0 + t, std::max<uint64_t>(n, 1) * sizeof(RunRef), &nx));
-    AMJ_TRY(scratch(ctx, SLOT_FIB_W, (n + 1) * 8, &w));
-    AMJ_TRY(scratch(ctx, SLOT_SCAN_TEMP, scan_temp_elems(n + 1) * 8, &tmp));
-    AMJ_HIP_TRY(ctx, launch_fib_weights(P, t, (uint64_t*)w, (RunRef*)nx, s));
-    AMJ_HIP_TRY(ctx, exclusive_scan_u64((uint64_t*)w, (uint64_t*)pw, n + 1, (uint64_t*)tmp, s));
-    P.tab[t].pw = (const uint64_t*)pw;
-    P.next[t] = (const RunRef*)nx;
-  }
-  const uint64_t n0 = P.probe_n;
-  void *off = nullptr, *pnx = nullptr;
-  AMJ_TRY(scratch(ctx, SLOT_PROBE_OFF, (n0 + 1) * 8, &off));
-  AMJ_TRY(scratch(ctx, SLOT_PROBE_NEXT, std::max<uint64_t>(n0, 1) * sizeof(RunRef), &pnx));
-  AMJ_TRY(scratch(ctx, SLOT_FIB_W, (n0 + 1) * 8, &w));
-  AMJ_TRY(scratch(ctx, SLOT_SCAN_TEMP, scan_temp_elems(n0 + 1) * 8, &tmp));
-  AMJ_HIP_TRY(ctx, launch_fib_probe_weights(P, (uint64_t*)w, (RunRef*)pnx, s));
-  AMJ_HIP_TRY(ctx, exclusive_scan_u64((uint64_t*)w, (uint64_t*)off, n0 + 1, (uint64_t*)tmp, s));
-  P.probe_next = (const RunRef*)pnx;
-  uint64_t total = 0;
-  AMJ_HIP_TRY(ctx, hipMemcpyAsync(&total, (uint64_t*)off + n0, 8, hipMemcpyDeviceToHost, s));
-  AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
-  res->count = total;
-
-  ProbeOut out{};
-  if (o->materialize && total > 0) {
-    const uint64_t need = total * (uint64_t)P.n_attrs * 4;
-    if (need > budget_left(ctx, o)) {
-      set_error(ctx, AMJ_ERR_NOMEM, "result of " + std::to_string(total) +
-                                        " tuples does not fit the HBM budget (use ooc=1 or "
-                                        "materialize=0)");
-      return AMJ_ERR_NOMEM;
-    }
-    AMJ_TRY(alloc_output(ctx, total, P.n_attrs, res));
-    for (int a = 0; a < P.n_attrs; ++a) out.cols[a] = res->cols[a];
-    out.capacity = total;
-  }
-  const uint64_t window = o->out_chunk_rows ? o->out_chunk_rows : kDefaultWindow;
-  const uint64_t n_windows = (total + window - 1) / window;
-  const uint64_t nb = window_blocks(total, window);
-  void *bsum = nullptr, *first = nullptr, *flag = nullptr;
-  AMJ_TRY(scratch(ctx, SLOT_BLOCK_SUMS, (nb + 1) * 8, &bsum));
-  AMJ_TRY(scratch(ctx, SLOT_MISC0, (n_windows + 1) * 8, &first));
-  AMJ_TRY(scratch(ctx, SLOT_FLAG, 256, &flag));
-  AMJ_HIP_TRY(ctx, hipMemsetAsync(flag, 0, 4, s));
-  uint64_t nb_used = 0;
-  const bool hash = o->checksum || !out.cols[0];
-  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
-  if (total > 0) {
-    AMJ_HIP_TRY(ctx, launch_window_write(P, (const uint64_t*)off, (uint64_t*)first, total, window,
-                                         out, hash, (uint64_t*)bsum, (uint32_t*)flag, &nb_used,
-                                         s));
-  } else {
-    nb_used = 1;
-    AMJ_HIP_TRY(ctx, hipMemsetAsync(bsum, 0, 8, s));
-  }
-  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
-  uint32_t overflow = 0;
-  AMJ_HIP_TRY(ctx, hipMemcpyAsync(&overflow, flag, 4, hipMemcpyDeviceToHost, s));
-  if (hash) AMJ_TRY(finish_checksum(ctx, (const uint64_t*)bsum, nb_used, &res->checksum));
-  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[5], s));
-  AMJ_HIP_TRY(ctx, hipEventSynchronize(ctx->ev[5]));
-  if (overflow) {
-    amj_result_free(ctx, res);
-    set_error(ctx, AMJ_ERR_STATE, "a frontier level exceeded 2^32 children");
-    return AMJ_ERR_STATE;
-  }
-  res->probe_ms = elapsed_ms(ctx->ev[2], ctx->ev[5]);
-  res->probe_kernel_ms = elapsed_ms(ctx->ev[3], ctx->ev[4]);
-  res->probe_kernel_launches = 1;
-  res->rows_written = out.cols[0] ? total : 0;
-  res->path = AMJ_PATH_FIB;
-  return AMJ_OK;
-}
-
 int execute_plan(amj_ctx_t* ctx, Plan& plan, const std::vector<uint32_t>& attr_ids,
                  const amj_opts_t* opts, amj_result_t* res) {
   amj_opts_t def;
@@ -447,7 +360,7 @@ int execute_plan(amj_ctx_t* ctx, Plan& plan, const std::vector<uint32_t>& attr_i
   std::memset(res, 0, sizeof(*res));
   res->num_cols = (uint32_t)plan.n_attrs;
   for (int a = 0; a < plan.n_attrs; ++a) res->attr_ids[a] = attr_ids[a];
-  if (o->fib && fib_eligible(plan)) {
+  if (o->fib && fib_eligible(plan) && window_plan_supported(plan)) {
     const int rc = run_fib(ctx, plan, o, res);
     if (rc != AMJ_ERR_STATE) return rc;
     std::memset(res, 0, sizeof(*res));  // extreme skew: fall back to the count/write path

@@ -62,6 +62,8 @@ enum ScratchSlot {
   SLOT_NEXT0,  // SLOT_NEXT0 + t: run pointers of table t into table t+1
   SLOT_NEXT_END = SLOT_NEXT0 + kMaxLevels,
   SLOT_PROBE_NEXT,
+  SLOT_SFX0,  // SLOT_SFX0 + t: suffix table of build table t
+  SLOT_SFX_END = SLOT_SFX0 + kMaxLevels,
   SLOT_FIB_W,
   SLOT_OOC_A,
   SLOT_OOC_B,
@@ -105,5 +107,14 @@ int execute_plan(amj_ctx_t* ctx, Plan& plan, const std::vector<uint32_t>& attr_i
 
 uint32_t floor_log2_u64(uint64_t v);
 double elapsed_ms(hipEvent_t a, hipEvent_t b);
+
+// Helpers shared by the execution drivers (runtime.cpp, fib.cpp).
+bool fib_eligible(const Plan& P);
+int alloc_output(amj_ctx_t* ctx, uint64_t rows, int n_attrs, amj_result_t* res);
+int finish_checksum(amj_ctx_t* ctx, const uint64_t* block_sums, uint64_t nb, uint64_t* out);
+uint64_t budget_left(amj_ctx_t* ctx, const amj_opts_t* o);
+// Frequent-item path: suffix tables + output-window writer. AMJ_ERR_STATE asks the caller to
+// use the count/write path instead (a frontier level would exceed 2^32 children).
+int run_fib(amj_ctx_t* ctx, Plan& P, const amj_opts_t* o, amj_result_t* res);
 
 }  // namespace amj

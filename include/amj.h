@@ -77,7 +77,8 @@ typedef struct {
   uint64_t out_chunk_rows;      /* dro: rows per output chunk (0: default) */
   uint64_t dro_capacity_rows;   /* dro: initial output allocation (0: sampled estimate) */
   uint32_t ws_split_threshold;  /* ws: split runs longer than this (0: default) */
-  uint32_t reserved[7];
+  int fib_suffix;               /* fib: 0 size-based suffix tables, 1 none, 2 deepest that fits */
+  uint32_t reserved[6];
 } amj_opts_t;
 
 /* Result of one join. Columns are device memory owned by the library until

@@ -15,8 +15,8 @@ pytestmark = pytest.mark.gpu
 
 GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "mhj_cpu_counts.json")))
 
-PATHS = [dict(), dict(fib=1)]
-PATH_IDS = ["plain", "fib"]
+PATHS = [dict(), dict(fib=1), dict(fib=1, fib_suffix=1), dict(fib=1, fib_suffix=2)]
+PATH_IDS = ["plain", "fib", "fib-nosuffix", "fib-deepsuffix"]
 
 
 def _rels(rec):
@@ -70,11 +70,12 @@ def test_fib_and_plain_write_identical_arrays(ctx):
     # both paths emit results in the same depth-first order over probe rows
     rels = G.chain_relations([20000, 5000, 7000, 3000], 300, seed=3, zipf_s=1.0)
     a = ctx.chain_join(rels, fetch=True)
-    b = ctx.chain_join(rels, fetch=True, fib=1, out_chunk_rows=1000)
-    assert a.path == "plain" and b.path == "fib"
-    assert a.count == b.count > 0
-    for ca, cb in zip(a.cols, b.cols):
-        assert np.array_equal(ca, cb)
+    for suffix in (0, 1, 2):
+        b = ctx.chain_join(rels, fetch=True, fib=1, out_chunk_rows=1000, fib_suffix=suffix)
+        assert a.path == "plain" and b.path == "fib"
+        assert a.count == b.count > 0
+        for ca, cb in zip(a.cols, b.cols):
+            assert np.array_equal(ca, cb)
 
 
 def test_heavy_skew_single_key(ctx):
