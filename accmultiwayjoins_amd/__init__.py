@@ -3,7 +3,7 @@
 The compute path is libamj.so (hand-written gfx950 HIP kernels behind a C ABI, see
 include/amj.h); this package is its host-side interface.
 """
-from .amj import AmjError, Context, HashTable, JoinResult, Relation  # noqa: F401
+from .amj import AmjError, Context, HashTable, JoinResult, Relation, Trie  # noqa: F401
 from . import generators  # noqa: F401
 
-__all__ = ["AmjError", "Context", "HashTable", "JoinResult", "Relation", "generators"]
+__all__ = ["AmjError", "Context", "HashTable", "JoinResult", "Relation", "Trie", "generators"]

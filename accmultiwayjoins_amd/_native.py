@@ -22,6 +22,7 @@ STATUS = {
 PATHS = {1: "plain", 2: "fib", 3: "ws", 4: "dro", 5: "lftj-dfs", 6: "lftj-bfs"}
 
 QUERY = {"NORMAL": 0, "TRI": 1, "FOUR": 2}
+ALGO = {"BFS": 0, "DFS": 1}
 
 
 class AmjRel(C.Structure):
@@ -97,6 +98,17 @@ SIGNATURES = [
     ("amj_result_free", C.c_int, [C.c_void_p, C.POINTER(AmjResult)]),
     ("amj_result_copy_col", C.c_int,
      [C.c_void_p, C.POINTER(AmjResult), C.c_uint32, C.c_uint64, C.c_uint64, C.c_void_p]),
+    ("alftj_trie_build", C.c_int, [C.c_void_p, C.POINTER(AmjRel), C.POINTER(C.c_void_p)]),
+    ("alftj_trie_destroy", None, [C.c_void_p, C.c_void_p]),
+    ("alftj_trie_info", C.c_int,
+     [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+      C.POINTER(C.c_uint64), C.POINTER(C.c_double)]),
+    ("alftj_trie_copy_level", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    ("alftj_run", C.c_int,
+     [C.c_void_p, C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(C.c_uint32), C.c_uint32, C.c_int,
+      C.POINTER(AmjOpts), C.POINTER(AmjResult)]),
+    ("alftj_pattern", C.c_int,
+     [C.c_void_p, C.POINTER(AmjRel), C.c_int, C.c_int, C.POINTER(AmjOpts), C.POINTER(AmjResult)]),
 ]
 
 GEN_SIGNATURES = [

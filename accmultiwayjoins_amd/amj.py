@@ -134,6 +134,44 @@ class HashTable:
             pass
 
 
+class Trie:
+    """Sorted two-level CSR trie of a two-column relation (IndexedTrie<D,C>,
+    LFTJ-GPU/IndexedTrie.cuh:21-54); duplicate pairs are dropped."""
+
+    def __init__(self, ctx: "Context", cols, attr_ids=(0, 1)):
+        keep = _RelKeep(cols, attr_ids)
+        h = C.c_void_p()
+        ctx._check(N.lib().alftj_trie_build(ctx.h, C.byref(keep.rel), C.byref(h)))
+        self.ctx, self.h, self.attr_ids = ctx, h, tuple(attr_ids)
+        n0, n1, mx, ms = C.c_uint64(), C.c_uint64(), C.c_uint64(), C.c_double()
+        ctx._check(N.lib().alftj_trie_info(ctx.h, h, C.byref(n0), C.byref(n1), C.byref(mx),
+                                           C.byref(ms)))
+        self.level0, self.children = n0.value, n1.value
+        self.max_children, self.build_ms = mx.value, ms.value
+
+    def levels(self):
+        """(level-0 values, level-0 offsets, level-1 values) as host arrays."""
+        d0 = np.empty(self.level0, dtype=np.int32)
+        off = np.empty(self.level0 + 1, dtype=np.uint32)
+        d1 = np.empty(self.children, dtype=np.int32)
+        self.ctx._check(N.lib().alftj_trie_copy_level(self.ctx.h, self.h, 0, d0.ctypes.data,
+                                                      off.ctypes.data))
+        self.ctx._check(N.lib().alftj_trie_copy_level(self.ctx.h, self.h, 1, d1.ctypes.data,
+                                                      None))
+        return d0, off, d1
+
+    def close(self):
+        if self.h:
+            N.lib().alftj_trie_destroy(self.ctx.h, self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Context:
     """One GPU, one HIP stream. Not shared between host threads."""
 
@@ -212,6 +250,30 @@ class Context:
         res = N.AmjResult()
         self._check(N.lib().amj_pattern_join(self.h, C.byref(keep.rel), N.QUERY[query],
                                              C.byref(o), C.byref(res)))
+        return self._finish(res, fetch)
+
+    def trie(self, cols, attr_ids=(0, 1)) -> Trie:
+        return Trie(self, cols, attr_ids)
+
+    def lftj(self, tries: Sequence[Trie], attr_order: Sequence[int], algo: str = "DFS",
+             fetch: bool = False, **opts) -> JoinResult:
+        """LFTJ_Base::evaluate over tries on attr_order; algo "DFS" or "BFS"."""
+        ts = (C.c_void_p * len(tries))(*[t.h.value for t in tries])
+        order = (C.c_uint32 * len(attr_order))(*attr_order)
+        o = _opts(**opts)
+        res = N.AmjResult()
+        self._check(N.lib().alftj_run(self.h, ts, len(tries), order, len(attr_order),
+                                      N.ALGO[algo], C.byref(o), C.byref(res)))
+        return self._finish(res, fetch)
+
+    def lftj_pattern(self, src, dst, query: str = "TRI", algo: str = "DFS", fetch: bool = False,
+                     **opts) -> JoinResult:
+        """exec-LFTJ SELF mode: TRI (or FOUR) over one edge relation E(src, dst)."""
+        keep = _RelKeep([src, dst], [0, 1])
+        o = _opts(**opts)
+        res = N.AmjResult()
+        self._check(N.lib().alftj_pattern(self.h, C.byref(keep.rel), N.QUERY[query],
+                                          N.ALGO[algo], C.byref(o), C.byref(res)))
         return self._finish(res, fetch)
 
     def relation(self, cols, attr_ids) -> Relation:

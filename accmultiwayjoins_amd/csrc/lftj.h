@@ -1,0 +1,72 @@
+// Leapfrog triejoin (ALFTJ) declarations: CSR tries and the query plan over them.
+#pragma once
+
+#include "amj_internal.h"
+
+namespace amj {
+
+constexpr int kMaxLftjAttrs = 4;  // triangle (3) and 4-clique (4)
+constexpr int kMaxLftjRels = 6;   // the 4-clique query joins six relations
+
+// Two-level sorted trie of a relation R(x, y) (IndexedTrie<D,C> with 2 attributes,
+// LFTJ-GPU/IndexedTrie.cuh:21-54): d0 = distinct x ascending, off[j] .. off[j+1] the children of
+// d0[j] in d1 (ascending, distinct).
+struct LRel {
+  const int32_t* d0;
+  const uint32_t* off;  // [n0 + 1]
+  const int32_t* d1;
+  const uint32_t* dense;  // optional: dense[v] = row of v in d0 (0xffffffff: absent)
+  uint32_t n0;
+  uint32_t dense_n;
+};
+
+struct TrieDev {
+  LRel view{};
+  int32_t* d0 = nullptr;
+  uint32_t* off = nullptr;
+  int32_t* d1 = nullptr;
+  uint32_t* dense = nullptr;
+  uint32_t n0 = 0, n1 = 0, dense_n = 0;
+  uint32_t max_run = 0;  // longest child list
+  double build_ms = 0.0;
+};
+
+// Query: attribute order 0..n_attrs-1 (the reference's attr_order, exec_LFTJ.cu:230-231);
+// relation r joins attributes x[r] < y[r].
+struct LPlan {
+  int32_t n_attrs;
+  int32_t n_rels;
+  LRel rel[kMaxLftjRels];
+  int32_t x[kMaxLftjRels];
+  int32_t y[kMaxLftjRels];
+  uint32_t root_mask[kMaxLftjAttrs];   // relations whose first attribute is i
+  uint32_t child_mask[kMaxLftjAttrs];  // relations whose second attribute is i
+  int32_t driver0;                     // root relation whose level 0 enumerates attribute 0
+  int32_t pad;
+  uint64_t roots;                      // level-0 size of driver0
+};
+
+int build_trie(amj_ctx_t* ctx, const int32_t* c0, const int32_t* c1, uint64_t n, TrieDev* out);
+void free_trie(amj_ctx_t* ctx, TrieDev* t);
+
+// DFS engine (lftj.hip): per-unit counts of 64 roots, then the write pass at scanned bases.
+uint64_t lftj_units(const LPlan& P);
+hipError_t launch_lftj_count(const LPlan& P, uint64_t* unit_counts, hipStream_t s);
+struct LOut {
+  int32_t* c[kMaxLftjAttrs];  // result columns (c[0] == nullptr: checksum only)
+};
+hipError_t launch_lftj_write(const LPlan& P, const uint64_t* unit_base, const LOut& cols,
+                             bool hash, uint64_t* block_sums, uint64_t* n_blocks, hipStream_t s);
+
+// BFS engine (lftj_bfs.hip): count and checksum, level by level within mem_budget bytes.
+int run_lftj_bfs(amj_ctx_t* ctx, const LPlan& P, uint64_t mem_budget, uint64_t* count,
+                 uint64_t* checksum);
+
+}  // namespace amj
+
+// A built trie as handed out by the C ABI: the device trie plus the attribute ids of its
+// two levels (IndexedTrie::attr_list, LFTJ-GPU/IndexedTrie.cuh:27).
+struct amj_trie {
+  amj::TrieDev dev;
+  uint32_t attr_ids[2] = {0, 1};
+};

@@ -1,0 +1,275 @@
+// Host driver of the leapfrog triejoin: trie objects, query plans over them, and the DFS / BFS
+// executions behind the alftj_* entry points.
+//
+// Reference: exec_LFTJ.cu:62-254 (query_processing: load or build the tries, SELF mode aliasing
+// trie 0 for the triangle, attr_order 0..n-1) and ALFTJ::evaluate (ALFTJ.cuh:1112-1304: build
+// the attribute -> relation map from the 2-level tries, pick the smallest level-0 relation,
+// count, allocate, write).
+#include <algorithm>
+#include <cstring>
+
+#include "lftj.h"
+#include "runtime.h"
+
+namespace amj {
+
+struct LUse {
+  const amj_trie* t;
+  uint32_t x_id, y_id;
+};
+
+// Attribute i of the plan is attr_order[i]; relation r joins positions x[r] < y[r].
+static int make_lplan(amj_ctx_t* ctx, const std::vector<LUse>& uses, const uint32_t* order,
+                      uint32_t na, LPlan* P) {
+  std::memset(P, 0, sizeof(*P));
+  if (na < 2 || na > (uint32_t)kMaxLftjAttrs) {
+    set_error(ctx, AMJ_ERR_ARG, "LFTJ queries need 2..4 attributes");
+    return AMJ_ERR_ARG;
+  }
+  if (uses.empty() || uses.size() > (size_t)kMaxLftjRels) {
+    set_error(ctx, AMJ_ERR_ARG, "LFTJ queries need 1..6 relations");
+    return AMJ_ERR_ARG;
+  }
+  auto pos_of = [&](uint32_t id) -> int {
+    for (uint32_t i = 0; i < na; ++i)
+      if (order[i] == id) return (int)i;
+    return -1;
+  };
+  for (uint32_t i = 0; i < na; ++i)
+    for (uint32_t j = i + 1; j < na; ++j)
+      if (order[i] == order[j]) {
+        set_error(ctx, AMJ_ERR_ARG, "attr_order repeats an attribute");
+        return AMJ_ERR_ARG;
+      }
+  P->n_attrs = (int32_t)na;
+  P->n_rels = (int32_t)uses.size();
+  uint64_t widest = 0;  // longest candidate list any frontier entry can take
+  for (size_t r = 0; r < uses.size(); ++r) {
+    const int px = pos_of(uses[r].x_id), py = pos_of(uses[r].y_id);
+    if (px < 0 || py < 0) {
+      set_error(ctx, AMJ_ERR_ARG, "a trie attribute is not in attr_order");
+      return AMJ_ERR_ARG;
+    }
+    if (px >= py) {
+      set_error(ctx, AMJ_ERR_ARG,
+                "trie levels must follow attr_order (build the trie with its columns swapped)");
+      return AMJ_ERR_ARG;
+    }
+    P->rel[r] = uses[r].t->dev.view;
+    P->x[r] = px;
+    P->y[r] = py;
+    P->root_mask[px] |= 1u << r;
+    P->child_mask[py] |= 1u << r;
+    widest = std::max<uint64_t>(widest, uses[r].t->dev.max_run);
+  }
+  if (P->root_mask[0] == 0u) {
+    set_error(ctx, AMJ_ERR_ARG, "no relation starts at the first attribute of attr_order");
+    return AMJ_ERR_ARG;
+  }
+  for (uint32_t i = 1; i < na; ++i) {
+    if ((P->root_mask[i] | P->child_mask[i]) == 0u) {
+      set_error(ctx, AMJ_ERR_ARG, "an attribute of attr_order is in no relation");
+      return AMJ_ERR_ARG;
+    }
+    if (P->child_mask[i] == 0u)  // candidates come from a level-0 list
+      for (size_t r = 0; r < uses.size(); ++r)
+        if ((P->root_mask[i] >> r) & 1u) widest = std::max<uint64_t>(widest, P->rel[r].n0);
+  }
+  // a wave's frontier level is indexed with 32 bits: 64 lists of at most 2^26 entries
+  if (widest >= (1ull << 26)) {
+    set_error(ctx, AMJ_ERR_ARG, "a candidate list exceeds 2^26 entries");
+    return AMJ_ERR_ARG;
+  }
+  // level 0 enumerates the smallest level-0 list among the relations rooted at attribute 0
+  // (ALFTJ.cuh:1188-1196)
+  int best = -1;
+  for (size_t r = 0; r < uses.size(); ++r)
+    if (((P->root_mask[0] >> r) & 1u) && (best < 0 || P->rel[r].n0 < P->rel[best].n0))
+      best = (int)r;
+  P->driver0 = best;
+  P->roots = P->rel[best].n0;
+  return AMJ_OK;
+}
+
+static int run_lftj_dfs(amj_ctx_t* ctx, const LPlan& P, const amj_opts_t* o, amj_result_t* res) {
+  hipStream_t s = ctx->stream;
+  const uint64_t units = lftj_units(P);
+  void *counts = nullptr, *tmp = nullptr, *bsum = nullptr;
+  AMJ_TRY(scratch(ctx, SLOT_UNIT_COUNTS, (units + 1) * 8, &counts));
+  AMJ_TRY(scratch(ctx, SLOT_SCAN_TEMP, scan_temp_elems(units + 1) * 8, &tmp));
+  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[2], s));
+  AMJ_HIP_TRY(ctx, hipMemsetAsync(counts, 0, (units + 1) * 8, s));
+  AMJ_HIP_TRY(ctx, launch_lftj_count(P, (uint64_t*)counts, s));
+  AMJ_HIP_TRY(ctx, exclusive_scan_u64((uint64_t*)counts, (uint64_t*)counts, units + 1,
+                                      (uint64_t*)tmp, s));
+  uint64_t total = 0;
+  AMJ_HIP_TRY(ctx, hipMemcpyAsync(&total, (uint64_t*)counts + units, 8, hipMemcpyDeviceToHost, s));
+  AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
+  res->count = total;
+  LOut out{};
+  if (o->materialize && total > 0) {
+    const uint64_t need = total * (uint64_t)P.n_attrs * 4;
+    if (need > budget_left(ctx, o)) {
+      set_error(ctx, AMJ_ERR_NOMEM, "result of " + std::to_string(total) +
+                                        " tuples does not fit the HBM budget (use "
+                                        "materialize=0)");
+      return AMJ_ERR_NOMEM;
+    }
+    AMJ_TRY(alloc_output(ctx, total, P.n_attrs, res));
+    for (int a = 0; a < P.n_attrs; ++a) out.c[a] = res->cols[a];
+  }
+  const uint64_t nb = units == 0 ? 1 : (units + kWavesPerBlock - 1) / kWavesPerBlock;
+  AMJ_TRY(scratch(ctx, SLOT_BLOCK_SUMS, nb * 8, &bsum));
+  const bool hash = o->checksum || !out.c[0];
+  uint64_t nb_used = 0;
+  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
+  AMJ_HIP_TRY(ctx, launch_lftj_write(P, (const uint64_t*)counts, out, hash, (uint64_t*)bsum,
+                                     &nb_used, s));
+  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
+  if (hash) AMJ_TRY(finish_checksum(ctx, (const uint64_t*)bsum, nb_used, &res->checksum));
+  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[5], s));
+  AMJ_HIP_TRY(ctx, hipEventSynchronize(ctx->ev[5]));
+  res->probe_ms = elapsed_ms(ctx->ev[2], ctx->ev[5]);
+  res->probe_kernel_ms = elapsed_ms(ctx->ev[3], ctx->ev[4]);
+  res->probe_kernel_launches = 1;
+  res->rows_written = out.c[0] ? total : 0;
+  res->path = AMJ_PATH_LFTJ_DFS;
+  return AMJ_OK;
+}
+
+int lftj_execute(amj_ctx_t* ctx, const std::vector<LUse>& uses, const uint32_t* order,
+                 uint32_t na, int algo, const amj_opts_t* opts, amj_result_t* res) {
+  amj_opts_t def;
+  amj_default_opts(&def);
+  const amj_opts_t* o = opts ? opts : &def;
+  std::memset(res, 0, sizeof(*res));
+  LPlan P;
+  AMJ_TRY(make_lplan(ctx, uses, order, na, &P));
+  res->num_cols = na;
+  for (uint32_t a = 0; a < na; ++a) res->attr_ids[a] = order[a];
+  if (algo == ALFTJ_DFS) return run_lftj_dfs(ctx, P, o, res);
+  if (algo != ALFTJ_BFS) {
+    set_error(ctx, AMJ_ERR_ARG, "algo must be ALFTJ_BFS (0) or ALFTJ_DFS (1)");
+    return AMJ_ERR_ARG;
+  }
+  // BFS: count and checksum (LFTJ_BFS::evaluate reports a count only, LFTJ_BFS.cuh:457-500)
+  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
+  AMJ_TRY(run_lftj_bfs(ctx, P, budget_left(ctx, o), &res->count, &res->checksum));
+  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[5], ctx->stream));
+  AMJ_HIP_TRY(ctx, hipEventSynchronize(ctx->ev[5]));
+  res->probe_ms = elapsed_ms(ctx->ev[2], ctx->ev[5]);
+  res->path = AMJ_PATH_LFTJ_BFS;
+  return AMJ_OK;
+}
+
+}  // namespace amj
+
+using namespace amj;
+
+extern "C" {
+
+int alftj_trie_build(amj_ctx_t* ctx, const amj_rel_t* rel, amj_trie_t** out) {
+  if (!ctx || !rel || !out) return AMJ_ERR_ARG;
+  *out = nullptr;
+  if (rel->num_cols != 2) {
+    set_error(ctx, AMJ_ERR_ARG, "a trie is built from a two-column relation");
+    return AMJ_ERR_ARG;
+  }
+  amj_relation_t* R = nullptr;
+  AMJ_TRY(amj_relation_create(ctx, rel, &R));
+  std::unique_ptr<amj_trie> t(new amj_trie());
+  t->attr_ids[0] = R->attr_ids[0];
+  t->attr_ids[1] = R->attr_ids[1];
+  const int rc = build_trie(ctx, R->cols[0], R->cols[1], R->rows, &t->dev);
+  amj_relation_destroy(ctx, R);
+  if (rc != AMJ_OK) {
+    free_trie(ctx, &t->dev);
+    return rc;
+  }
+  *out = t.release();
+  return AMJ_OK;
+}
+
+void alftj_trie_destroy(amj_ctx_t* ctx, amj_trie_t* t) {
+  if (!t) return;
+  free_trie(ctx, &t->dev);
+  delete t;
+}
+
+int alftj_trie_info(amj_ctx_t* ctx, const amj_trie_t* t, uint64_t* level0, uint64_t* children,
+                    uint64_t* max_children, double* build_ms) {
+  if (!ctx || !t) return AMJ_ERR_ARG;
+  if (level0) *level0 = t->dev.n0;
+  if (children) *children = t->dev.n1;
+  if (max_children) *max_children = t->dev.max_run;
+  if (build_ms) *build_ms = t->dev.build_ms;
+  return AMJ_OK;
+}
+
+int alftj_trie_copy_level(amj_ctx_t* ctx, const amj_trie_t* t, int level, int32_t* values,
+                          uint32_t* offsets) {
+  if (!ctx || !t || (level != 0 && level != 1)) return AMJ_ERR_ARG;
+  hipStream_t s = ctx->stream;
+  if (level == 0) {
+    if (values && t->dev.n0)
+      AMJ_HIP_TRY(ctx, hipMemcpyAsync(values, t->dev.d0, (uint64_t)t->dev.n0 * 4,
+                                      hipMemcpyDeviceToHost, s));
+    if (offsets)
+      AMJ_HIP_TRY(ctx, hipMemcpyAsync(offsets, t->dev.off, ((uint64_t)t->dev.n0 + 1) * 4,
+                                      hipMemcpyDeviceToHost, s));
+  } else if (values && t->dev.n1) {
+    AMJ_HIP_TRY(ctx, hipMemcpyAsync(values, t->dev.d1, (uint64_t)t->dev.n1 * 4,
+                                    hipMemcpyDeviceToHost, s));
+  }
+  AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
+  return AMJ_OK;
+}
+
+int alftj_run(amj_ctx_t* ctx, amj_trie_t* const* tries, uint32_t n, const uint32_t* attr_order,
+              uint32_t n_attrs, int algo, const amj_opts_t* opts, amj_result_t* res) {
+  if (!ctx || !tries || !attr_order || !res) return AMJ_ERR_ARG;
+  std::vector<LUse> uses;
+  for (uint32_t r = 0; r < n; ++r) {
+    if (!tries[r]) return AMJ_ERR_ARG;
+    uses.push_back(LUse{tries[r], tries[r]->attr_ids[0], tries[r]->attr_ids[1]});
+  }
+  return lftj_execute(ctx, uses, attr_order, n_attrs, algo, opts, res);
+}
+
+int alftj_pattern(amj_ctx_t* ctx, const amj_rel_t* edges, int query, int algo,
+                  const amj_opts_t* opts, amj_result_t* res) {
+  if (!ctx || !edges || !res) return AMJ_ERR_ARG;
+  std::memset(res, 0, sizeof(*res));
+  // SELF mode (exec_LFTJ.cu:198-222): one trie, aliased as (0,1), (1,2), (0,2) for the
+  // triangle; the 4-clique adds (2,3), (0,3), (1,3)
+  static const uint32_t tri[][2] = {{0, 1}, {1, 2}, {0, 2}};
+  static const uint32_t four[][2] = {{0, 1}, {1, 2}, {2, 3}, {0, 3}, {0, 2}, {1, 3}};
+  const uint32_t(*q)[2] = nullptr;
+  uint32_t nq = 0, na = 0;
+  if (query == AMJ_QUERY_TRI) {
+    q = tri;
+    nq = 3;
+    na = 3;
+  } else if (query == AMJ_QUERY_FOUR) {
+    q = four;
+    nq = 6;
+    na = 4;
+  } else {
+    set_error(ctx, AMJ_ERR_ARG, "pattern query must be TRI or FOUR");
+    return AMJ_ERR_ARG;
+  }
+  amj_rel_t e = *edges;
+  const uint32_t ids01[2] = {0, 1};
+  e.attr_ids = ids01;
+  amj_trie_t* T = nullptr;
+  AMJ_TRY(alftj_trie_build(ctx, &e, &T));
+  std::vector<LUse> uses;
+  for (uint32_t i = 0; i < nq; ++i) uses.push_back(LUse{T, q[i][0], q[i][1]});
+  const uint32_t order[4] = {0, 1, 2, 3};
+  const int rc = lftj_execute(ctx, uses, order, na, algo, opts, res);
+  if (rc == AMJ_OK) res->build_ms = T->dev.build_ms;
+  alftj_trie_destroy(ctx, T);
+  return rc;
+}
+
+}  // extern "C"

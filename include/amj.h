@@ -145,6 +145,31 @@ int amj_result_free(amj_ctx_t* ctx, amj_result_t* res);
 int amj_result_copy_col(amj_ctx_t* ctx, const amj_result_t* res, uint32_t col, uint64_t row0,
                         uint64_t n, int32_t* host_out);
 
+/* ---- ALFTJ (leapfrog triejoin) ---------------------------------------------------------- */
+/* Sorted two-level CSR trie of a two-column relation (constructSortedTrie<D,C,2>,
+ * LFTJ-GPU/multi_partitioning.cuh:129-293, IndexedTrie, LFTJ-GPU/IndexedTrie.cuh:21-54):
+ * level 0 = distinct col0 values, level 1 = their sorted distinct col1 children. Duplicate
+ * pairs are dropped (the triejoin intersects sets). attr_ids of `rel` name the two levels. */
+int alftj_trie_build(amj_ctx_t* ctx, const amj_rel_t* rel, amj_trie_t** out);
+void alftj_trie_destroy(amj_ctx_t* ctx, amj_trie_t* trie);
+/* Level sizes, the longest child list and the build time of a trie. */
+int alftj_trie_info(amj_ctx_t* ctx, const amj_trie_t* trie, uint64_t* level0, uint64_t* children,
+                    uint64_t* max_children, double* build_ms);
+/* Copy a level to host memory: level 0 values (n0) and offsets (n0 + 1), or level 1 values. */
+int alftj_trie_copy_level(amj_ctx_t* ctx, const amj_trie_t* trie, int level, int32_t* values,
+                          uint32_t* offsets);
+/* LFTJ_Base::evaluate via OOCWrapper::execute_with_{DFS,BFS} (LFTJ-GPU/LFTJ_base.h:20-22,
+ * LFTJ-GPU/ooc_wrapper.cuh:404-427): join the tries on attributes attr_order[0..n_attrs-1]
+ * (2..4 attributes, 1..6 tries; each trie's level-0 attribute precedes its level-1 attribute
+ * in attr_order). Output column i holds attribute attr_order[i]. algo = ALFTJ_DFS
+ * materialises (opts->materialize) and checksums; ALFTJ_BFS returns count and checksum. */
+int alftj_run(amj_ctx_t* ctx, amj_trie_t* const* tries, uint32_t n, const uint32_t* attr_order,
+              uint32_t n_attrs, int algo, const amj_opts_t* opts, amj_result_t* res);
+/* exec-LFTJ SELF mode (exec_LFTJ.cu:198-231): one trie over `edges`, aliased as R(0,1),
+ * S(1,2), T(0,2) for TRI; FOUR adds (2,3), (0,3), (1,3) (the 4-clique of exec_AMHJ.cu:59). */
+int alftj_pattern(amj_ctx_t* ctx, const amj_rel_t* edges, int query, int algo,
+                  const amj_opts_t* opts, amj_result_t* res);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,159 @@
+"""GPU parity of the leapfrog triejoin (ALFTJ DFS and BFS) and of the trie build.
+
+Pins (no reference run): the triangle KAT of LFTJ-GPU/uniform_writer.cu:18-19 (5 results),
+brute-force natural joins of small deduplicated relations, the CPU oracle's triangle and
+4-clique counts and checksums on R-MAT graphs, closed forms C(n,3) / C(n,4) on K_n, the AMHJ
+TRI/FOUR path on the same edges, and BFS == DFS.
+"""
+from math import comb
+
+import numpy as np
+import pytest
+
+import oracle_ffi as O
+from accmultiwayjoins_amd import AmjError
+from accmultiwayjoins_amd import generators as G
+
+pytestmark = pytest.mark.gpu
+
+ALGOS = ["DFS", "BFS"]
+
+
+def _tuples(r):
+    return [tuple(int(x) for x in t) for t in O.sort_tuples(r.tuples())]
+
+
+def _dedupe(cols):
+    p = np.unique(np.stack([np.asarray(c, dtype=np.int64) for c in cols], axis=1), axis=0)
+    return [p[:, 0].astype(np.int32), p[:, 1].astype(np.int32)]
+
+
+def test_kat_triangle(ctx):
+    from test_oracle import KAT_TRIANGLE, kat_relations
+    tries = [ctx.trie(cols, ids) for cols, ids in kat_relations()]
+    r = ctx.lftj(tries, [0, 1, 2], "DFS", fetch=True)
+    assert r.count == 5 and r.path == "lftj-dfs"
+    assert _tuples(r) == KAT_TRIANGLE
+    assert r.checksum == O.checksum(KAT_TRIANGLE)
+    b = ctx.lftj(tries, [0, 1, 2], "BFS")
+    assert (b.count, b.checksum, b.path) == (5, r.checksum, "lftj-bfs")
+
+
+def test_trie_levels_closed_form(ctx):
+    rng = np.random.default_rng(3)
+    n = 20000
+    c0 = rng.integers(-50, 3000, n).astype(np.int32)  # negatives: no dense index
+    c1 = rng.integers(-9, 40, n).astype(np.int32)
+    c0[:500] = c0[500:1000]  # duplicate pairs
+    c1[:500] = c1[500:1000]
+    for cols in ([c0, c1], [np.abs(c0), c1]):  # second: dense level-0 index
+        t = ctx.trie(cols)
+        d0, off, d1 = t.levels()
+        u = _dedupe(cols)
+        want_d0, first = np.unique(u[0], return_index=True)
+        assert np.array_equal(d0, want_d0)
+        assert np.array_equal(off, np.append(first, len(u[0])).astype(np.uint32))
+        assert np.array_equal(d1, u[1])
+        assert t.max_children == int(np.diff(off).max())
+        t.close()
+
+
+def test_trie_empty_and_single(ctx):
+    e = ctx.trie([np.zeros(0, np.int32), np.zeros(0, np.int32)])
+    assert (e.level0, e.children) == (0, 0)
+    one = ctx.trie([np.array([7], np.int32), np.array([7], np.int32)])
+    assert (one.level0, one.children, one.max_children) == (1, 1, 1)
+    for algo in ALGOS:
+        r = ctx.lftj([e, one], [0, 1], algo)
+        assert r.count == 0
+        r = ctx.lftj([one, one, one], [0, 1, 2], algo)  # (7,7,7)
+        assert r.count == 1 and r.checksum == O.checksum([(7, 7, 7)])
+
+
+QUERIES = {
+    "path": [(0, 1), (1, 2)],
+    "triangle": [(0, 1), (1, 2), (0, 2)],
+    "star": [(0, 1), (0, 2), (0, 3)],
+    "cycle4": [(0, 1), (1, 2), (2, 3), (0, 3)],
+    "clique4": [(0, 1), (1, 2), (2, 3), (0, 3), (0, 2), (1, 3)],
+    "product": [(0, 1), (2, 3)],  # attribute 2 has no parent: level-0 candidates
+    "two": [(0, 1), (0, 1)],
+}
+
+
+@pytest.mark.parametrize("name", sorted(QUERIES))
+@pytest.mark.parametrize("algo", ALGOS)
+def test_small_queries_equal_brute_force(ctx, name, algo):
+    rng = np.random.default_rng(len(name))
+    rels = []
+    for x, y in QUERIES[name]:
+        n = int(rng.integers(50, 300))
+        cols = _dedupe([rng.integers(0, 12, n), rng.integers(0, 12, n)])
+        rels.append((cols, [x, y]))
+    want = O.brute_join(rels)
+    A = 1 + max(max(q) for q in QUERIES[name])
+    tries = [ctx.trie(c, ids) for c, ids in rels]
+    r = ctx.lftj(tries, list(range(A)), algo, fetch=True)
+    assert r.count == len(want)
+    assert r.checksum == O.checksum(want)
+    if algo == "DFS":
+        assert _tuples(r) == want
+
+
+@pytest.mark.parametrize("scale", [8, 11, 14])
+@pytest.mark.parametrize("algo", ALGOS)
+def test_tri_four_match_graph_oracle(ctx, scale, algo):
+    s, d = G.rmat_oriented(scale, 16, seed=42)
+    n3, c3 = O.triangles(s, d)
+    r3 = ctx.lftj_pattern(s, d, "TRI", algo, materialize=False)
+    assert (r3.count, r3.checksum) == (n3, c3)
+    if scale <= 11:
+        n4, c4 = O.four_cliques(s, d)
+        r4 = ctx.lftj_pattern(s, d, "FOUR", algo, materialize=False)
+        assert (r4.count, r4.checksum) == (n4, c4)
+
+
+def test_tri_materialized_and_amhj_agree(ctx):
+    s, d = G.rmat_oriented(12, 16, seed=7)
+    n3, c3 = O.triangles(s, d)
+    r = ctx.lftj_pattern(s, d, "TRI", "DFS", fetch=True)
+    assert r.count == n3 and r.rows_written == n3
+    t = r.tuples()
+    assert O.checksum(t) == c3
+    h = ctx.pattern_join(s, d, "TRI", materialize=False)
+    assert (h.count, h.checksum) == (n3, c3)
+    h4 = ctx.pattern_join(s, d, "FOUR", materialize=False)
+    l4 = ctx.lftj_pattern(s, d, "FOUR", "DFS", materialize=False)
+    assert (h4.count, h4.checksum) == (l4.count, l4.checksum)
+
+
+@pytest.mark.parametrize("n", [5, 30, 64, 65])
+def test_complete_graph_closed_forms(ctx, n):
+    a, b = np.triu_indices(n, 1)
+    s, d = a.astype(np.int32), b.astype(np.int32)
+    for algo in ALGOS:
+        assert ctx.lftj_pattern(s, d, "TRI", algo, materialize=False).count == comb(n, 3)
+        assert ctx.lftj_pattern(s, d, "FOUR", algo, materialize=False).count == comb(n, 4)
+
+
+def test_duplicate_edges_are_ignored(ctx):
+    s, d = G.rmat_oriented(9, 16, seed=1)
+    n3, c3 = O.triangles(s, d)
+    s2, d2 = np.concatenate([s, s[::3]]), np.concatenate([d, d[::3]])
+    r = ctx.lftj_pattern(s2, d2, "TRI", "DFS", materialize=False)
+    assert (r.count, r.checksum) == (n3, c3)
+
+
+def test_attr_order_errors(ctx):
+    t01 = ctx.trie([np.array([1, 2], np.int32), np.array([2, 3], np.int32)], (0, 1))
+    t21 = ctx.trie([np.array([1, 2], np.int32), np.array([2, 3], np.int32)], (2, 1))
+    with pytest.raises(AmjError, match="follow attr_order"):
+        ctx.lftj([t01, t21], [0, 1, 2])
+    with pytest.raises(AmjError, match="not in attr_order"):
+        ctx.lftj([t01], [0, 2])
+    with pytest.raises(AmjError, match="2..4 attributes"):
+        ctx.lftj([t01], [0, 1, 2, 3, 4])
+    with pytest.raises(AmjError, match="in no relation"):
+        ctx.lftj([t01], [0, 1, 2])
+    # the context stays usable after errors
+    assert ctx.lftj([t01, t01], [0, 1]).count == 2
