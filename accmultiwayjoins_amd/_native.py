@@ -107,8 +107,24 @@ SIGNATURES = [
     ("alftj_run", C.c_int,
      [C.c_void_p, C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(C.c_uint32), C.c_uint32, C.c_int,
       C.POINTER(AmjOpts), C.POINTER(AmjResult)]),
+    ("alftj_run_aliased", C.c_int,
+     [C.c_void_p, C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(C.c_uint32),
+      C.POINTER(C.c_uint32), C.c_uint32, C.c_int, C.POINTER(AmjOpts), C.POINTER(AmjResult)]),
     ("alftj_pattern", C.c_int,
      [C.c_void_p, C.POINTER(AmjRel), C.c_int, C.c_int, C.POINTER(AmjOpts), C.POINTER(AmjResult)]),
+    ("alftj_trie_from_csr", C.c_int,
+     [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,
+      C.POINTER(C.c_void_p)]),
+    ("alftj_trie_save", C.c_int, [C.c_void_p, C.c_void_p, C.c_char_p]),
+    ("alftj_trie_load", C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_void_p)]),
+    ("amj_db_parse_name", C.c_int,
+     [C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+      C.c_uint32]),
+    ("amj_db_read", C.c_int, [C.c_char_p, C.c_uint64, C.c_uint32, C.POINTER(C.c_void_p)]),
+    ("amj_db_write", C.c_int, [C.c_char_p, C.POINTER(AmjRel), C.c_char_p, C.c_size_t]),
+    ("amj_file_error", C.c_char_p, []),
+    ("amj_result_md5", C.c_int, [C.c_void_p, C.POINTER(AmjResult), C.c_char_p]),
+    ("amj_res_stat_md5", C.c_int, [C.POINTER(C.c_void_p), C.c_uint32, C.c_uint64, C.c_char_p]),
 ]
 
 GEN_SIGNATURES = [

@@ -138,16 +138,23 @@ class Trie:
     """Sorted two-level CSR trie of a two-column relation (IndexedTrie<D,C>,
     LFTJ-GPU/IndexedTrie.cuh:21-54); duplicate pairs are dropped."""
 
-    def __init__(self, ctx: "Context", cols, attr_ids=(0, 1)):
-        keep = _RelKeep(cols, attr_ids)
-        h = C.c_void_p()
-        ctx._check(N.lib().alftj_trie_build(ctx.h, C.byref(keep.rel), C.byref(h)))
+    def __init__(self, ctx: "Context", cols=None, attr_ids=(0, 1), handle=None):
+        if handle is None:
+            keep = _RelKeep(cols, attr_ids)
+            h = C.c_void_p()
+            ctx._check(N.lib().alftj_trie_build(ctx.h, C.byref(keep.rel), C.byref(h)))
+        else:
+            h = handle
         self.ctx, self.h, self.attr_ids = ctx, h, tuple(attr_ids)
         n0, n1, mx, ms = C.c_uint64(), C.c_uint64(), C.c_uint64(), C.c_double()
         ctx._check(N.lib().alftj_trie_info(ctx.h, h, C.byref(n0), C.byref(n1), C.byref(mx),
                                            C.byref(ms)))
         self.level0, self.children = n0.value, n1.value
         self.max_children, self.build_ms = mx.value, ms.value
+
+    def save(self, path: str):
+        """Write the trie as a .tr file (IndexedTrie::serialization layout)."""
+        self.ctx._check(N.lib().alftj_trie_save(self.ctx.h, self.h, path.encode()))
 
     def levels(self):
         """(level-0 values, level-0 offsets, level-1 values) as host arrays."""
@@ -255,15 +262,52 @@ class Context:
     def trie(self, cols, attr_ids=(0, 1)) -> Trie:
         return Trie(self, cols, attr_ids)
 
+    def trie_from_csr(self, d0, off, d1, attr_ids=(0, 1)) -> Trie:
+        """A trie from CSR arrays (level-0 values, u64 offsets, children)."""
+        d0 = np.ascontiguousarray(d0, dtype=np.int32)
+        off = np.ascontiguousarray(off, dtype=np.uint64)
+        d1 = np.ascontiguousarray(d1, dtype=np.int32)
+        if len(off) != len(d0) + 1:
+            raise ValueError("offsets must have len(d0) + 1 entries")
+        ids = (C.c_uint32 * 2)(*attr_ids)
+        h = C.c_void_p()
+        self._check(N.lib().alftj_trie_from_csr(self.h, len(d0), d0.ctypes.data, off.ctypes.data,
+                                                len(d1), d1.ctypes.data, ids, C.byref(h)))
+        return Trie(self, attr_ids=tuple(attr_ids), handle=h)
+
+    def load_trie(self, path: str) -> Trie:
+        """Read a two-level .tr file."""
+        h = C.c_void_p()
+        self._check(N.lib().alftj_trie_load(self.h, path.encode(), C.byref(h)))
+        with open(path, "rb") as f:
+            hdr = f.read(4 + 16 + 8)
+        ids = tuple(np.frombuffer(hdr[20:28], dtype=np.uint32).tolist())
+        return Trie(self, attr_ids=ids, handle=h)
+
+    def md5(self, r: JoinResult) -> str:
+        """res_stat fingerprint of a materialised result still held on the device."""
+        if r._res is None:
+            raise ValueError("the result has been freed (fetch=True frees it)")
+        out = C.create_string_buffer(33)
+        self._check(N.lib().amj_result_md5(self.h, C.byref(r._res), out))
+        return out.value.decode()
+
     def lftj(self, tries: Sequence[Trie], attr_order: Sequence[int], algo: str = "DFS",
-             fetch: bool = False, **opts) -> JoinResult:
-        """LFTJ_Base::evaluate over tries on attr_order; algo "DFS" or "BFS"."""
+             fetch: bool = False, trie_attrs=None, **opts) -> JoinResult:
+        """LFTJ_Base::evaluate over tries on attr_order; algo "DFS" or "BFS". trie_attrs
+        ([(x, y), ...], one pair per trie) overrides the tries' own attribute ids."""
         ts = (C.c_void_p * len(tries))(*[t.h.value for t in tries])
         order = (C.c_uint32 * len(attr_order))(*attr_order)
         o = _opts(**opts)
         res = N.AmjResult()
-        self._check(N.lib().alftj_run(self.h, ts, len(tries), order, len(attr_order),
-                                      N.ALGO[algo], C.byref(o), C.byref(res)))
+        if trie_attrs is None:
+            rc = N.lib().alftj_run(self.h, ts, len(tries), order, len(attr_order),
+                                   N.ALGO[algo], C.byref(o), C.byref(res))
+        else:
+            flat = (C.c_uint32 * (2 * len(tries)))(*[int(a) for p in trie_attrs for a in p])
+            rc = N.lib().alftj_run_aliased(self.h, ts, len(tries), flat, order, len(attr_order),
+                                           N.ALGO[algo], C.byref(o), C.byref(res))
+        self._check(rc)
         return self._finish(res, fetch)
 
     def lftj_pattern(self, src, dst, query: str = "TRI", algo: str = "DFS", fetch: bool = False,

@@ -47,6 +47,8 @@ struct LPlan {
 };
 
 int build_trie(amj_ctx_t* ctx, const int32_t* c0, const int32_t* c1, uint64_t n, TrieDev* out);
+int trie_from_host(amj_ctx_t* ctx, uint64_t n0, const int32_t* d0, const uint64_t* off,
+                   uint64_t n1, const int32_t* d1, TrieDev* out);
 void free_trie(amj_ctx_t* ctx, TrieDev* t);
 
 // DFS engine (lftj.hip): per-unit counts of 64 roots, then the write pass at scanned bases.

@@ -236,6 +236,18 @@ int alftj_run(amj_ctx_t* ctx, amj_trie_t* const* tries, uint32_t n, const uint32
   return lftj_execute(ctx, uses, attr_order, n_attrs, algo, opts, res);
 }
 
+int alftj_run_aliased(amj_ctx_t* ctx, amj_trie_t* const* tries, uint32_t n,
+                      const uint32_t* trie_attrs, const uint32_t* attr_order, uint32_t n_attrs,
+                      int algo, const amj_opts_t* opts, amj_result_t* res) {
+  if (!ctx || !tries || !trie_attrs || !attr_order || !res) return AMJ_ERR_ARG;
+  std::vector<LUse> uses;
+  for (uint32_t r = 0; r < n; ++r) {
+    if (!tries[r]) return AMJ_ERR_ARG;
+    uses.push_back(LUse{tries[r], trie_attrs[2 * r], trie_attrs[2 * r + 1]});
+  }
+  return lftj_execute(ctx, uses, attr_order, n_attrs, algo, opts, res);
+}
+
 int alftj_pattern(amj_ctx_t* ctx, const amj_rel_t* edges, int query, int algo,
                   const amj_opts_t* opts, amj_result_t* res) {
   if (!ctx || !edges || !res) return AMJ_ERR_ARG;

@@ -102,6 +102,41 @@ static unsigned grid_for(uint64_t n) {
   return g ? (unsigned)g : 1u;
 }
 
+// Longest child run, optional dense level-0 index, timing and the device view. Expects d0,
+// off, d1, n0 and n1 set; ev[0] recorded at the start of the build.
+static int finish_trie(amj_ctx_t* ctx, TrieDev* out) {
+  hipStream_t s = ctx->stream;
+  const uint32_t n0 = out->n0;
+  uint32_t max_run = 0;
+  if (n0) {
+    void* mr = nullptr;
+    AMJ_TRY(scratch(ctx, SLOT_FLAG, 256, &mr));
+    AMJ_HIP_TRY(ctx, hipMemsetAsync(mr, 0, 4, s));
+    hipLaunchKernelGGL(trie_max_run_kernel, dim3(std::min(grid_for(n0), 1024u)), dim3(kBlock), 0,
+                       s, (const uint32_t*)out->off, n0, (uint32_t*)mr);
+    AMJ_HIP_TRY(ctx, hipMemcpyAsync(&max_run, mr, 4, hipMemcpyDeviceToHost, s));
+    // dense value -> row index when level-0 values are small non-negative integers
+    int32_t lo_v = 0, hi_v = 0;
+    AMJ_HIP_TRY(ctx, hipMemcpyAsync(&lo_v, out->d0, 4, hipMemcpyDeviceToHost, s));
+    AMJ_HIP_TRY(ctx, hipMemcpyAsync(&hi_v, out->d0 + n0 - 1, 4, hipMemcpyDeviceToHost, s));
+    AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
+    const uint64_t span = (uint64_t)(uint32_t)hi_v + 1;
+    if (lo_v >= 0 && span <= std::max<uint64_t>(4ull * n0, 1ull << 20)) {
+      AMJ_TRY(dmalloc(ctx, (void**)&out->dense, span * 4));
+      out->dense_n = (uint32_t)span;
+      AMJ_HIP_TRY(ctx, hipMemsetAsync(out->dense, 0xff, span * 4, s));
+      hipLaunchKernelGGL(trie_dense_kernel, dim3(grid_for(n0)), dim3(kBlock), 0, s,
+                         (const int32_t*)out->d0, n0, out->dense);
+    }
+  }
+  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[1], s));
+  AMJ_HIP_TRY(ctx, hipEventSynchronize(ctx->ev[1]));
+  out->build_ms = elapsed_ms(ctx->ev[0], ctx->ev[1]);
+  out->max_run = max_run;
+  out->view = LRel{out->d0, out->off, out->d1, out->dense, n0, out->dense_n};
+  return AMJ_OK;
+}
+
 int build_trie(amj_ctx_t* ctx, const int32_t* c0, const int32_t* c1, uint64_t n,
                TrieDev* out) {
   *out = TrieDev{};
@@ -112,11 +147,9 @@ int build_trie(amj_ctx_t* ctx, const int32_t* c0, const int32_t* c1, uint64_t n,
   hipStream_t s = ctx->stream;
   AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[0], s));
   if (n == 0) {
-    AMJ_TRY(dmalloc(ctx, (void**)&out->off, 8));
-    AMJ_HIP_TRY(ctx, hipMemsetAsync(out->off, 0, 8, s));
-    AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
-    out->view.off = out->off;
-    return AMJ_OK;
+    AMJ_TRY(dmalloc(ctx, (void**)&out->off, 4));
+    AMJ_HIP_TRY(ctx, hipMemsetAsync(out->off, 0, 4, s));
+    return finish_trie(ctx, out);
   }
   void *keys = nullptr, *alt = nullptr, *tmp = nullptr, *keep = nullptr, *pos = nullptr,
        *head = nullptr, *hpos = nullptr, *stmp = nullptr;
@@ -139,6 +172,7 @@ int build_trie(amj_ctx_t* ctx, const int32_t* c0, const int32_t* c1, uint64_t n,
   AMJ_HIP_TRY(ctx, hipMemcpyAsync(&m, (uint32_t*)pos + n, 4, hipMemcpyDeviceToHost, s));
   AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
   AMJ_TRY(dmalloc(ctx, (void**)&out->d1, (uint64_t)m * 4));
+  out->n1 = m;
   hipLaunchKernelGGL(trie_compact_kernel, dim3(grid_for(n)), dim3(kBlock), 0, s,
                      (const uint64_t*)keys, n, (const uint32_t*)pos, (const uint32_t*)keep,
                      out->d1, (uint32_t*)head);
@@ -149,39 +183,54 @@ int build_trie(amj_ctx_t* ctx, const int32_t* c0, const int32_t* c1, uint64_t n,
   AMJ_HIP_TRY(ctx, hipMemcpyAsync(&n0, (uint32_t*)hpos + m, 4, hipMemcpyDeviceToHost, s));
   AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
   AMJ_TRY(dmalloc(ctx, (void**)&out->d0, (uint64_t)n0 * 4));
+  out->n0 = n0;
   AMJ_TRY(dmalloc(ctx, (void**)&out->off, ((uint64_t)n0 + 1) * 4));
   hipLaunchKernelGGL(trie_level0_kernel, dim3(grid_for(n)), dim3(kBlock), 0, s,
                      (const uint64_t*)keys, n, (const uint32_t*)pos, (const uint32_t*)keep,
                      (const uint32_t*)head, (const uint32_t*)hpos, out->d0, out->off);
   AMJ_HIP_TRY(ctx, hipMemcpyAsync(out->off + n0, &m, 4, hipMemcpyHostToDevice, s));
-  void* mr = nullptr;
-  AMJ_TRY(scratch(ctx, SLOT_FLAG, 256, &mr));
-  AMJ_HIP_TRY(ctx, hipMemsetAsync(mr, 0, 4, s));
-  hipLaunchKernelGGL(trie_max_run_kernel, dim3(std::min(grid_for(n0), 1024u)), dim3(kBlock), 0, s,
-                     (const uint32_t*)out->off, n0, (uint32_t*)mr);
-  uint32_t max_run = 0;
-  AMJ_HIP_TRY(ctx, hipMemcpyAsync(&max_run, mr, 4, hipMemcpyDeviceToHost, s));
-  // dense value -> row index when level-0 values are small non-negative integers
-  int32_t lo_v = 0, hi_v = 0;
-  AMJ_HIP_TRY(ctx, hipMemcpyAsync(&lo_v, out->d0, 4, hipMemcpyDeviceToHost, s));
-  AMJ_HIP_TRY(ctx, hipMemcpyAsync(&hi_v, out->d0 + n0 - 1, 4, hipMemcpyDeviceToHost, s));
-  AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
-  const uint64_t span = (uint64_t)(uint32_t)hi_v + 1;
-  if (lo_v >= 0 && span <= std::max<uint64_t>(4ull * n0, 1ull << 20)) {
-    AMJ_TRY(dmalloc(ctx, (void**)&out->dense, span * 4));
-    AMJ_HIP_TRY(ctx, hipMemsetAsync(out->dense, 0xff, span * 4, s));
-    hipLaunchKernelGGL(trie_dense_kernel, dim3(grid_for(n0)), dim3(kBlock), 0, s,
-                       (const int32_t*)out->d0, n0, out->dense);
-    out->dense_n = (uint32_t)span;
+  return finish_trie(ctx, out);
+}
+
+// Host CSR (e.g. a deserialised .tr file) -> device trie. Level 0 must be strictly increasing
+// and every child run sorted (duplicate children are kept, as in the reference's tries).
+int trie_from_host(amj_ctx_t* ctx, uint64_t n0, const int32_t* d0, const uint64_t* off,
+                   uint64_t n1, const int32_t* d1, TrieDev* out) {
+  *out = TrieDev{};
+  if (n0 >= (1ull << 32) || n1 >= (1ull << 32)) {
+    set_error(ctx, AMJ_ERR_ARG, "trie levels must hold fewer than 2^32 values");
+    return AMJ_ERR_ARG;
   }
-  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[1], s));
-  AMJ_HIP_TRY(ctx, hipEventSynchronize(ctx->ev[1]));
-  out->build_ms = elapsed_ms(ctx->ev[0], ctx->ev[1]);
-  out->n0 = n0;
-  out->n1 = m;
-  out->max_run = max_run;
-  out->view = LRel{out->d0, out->off, out->d1, out->dense, n0, out->dense_n};
-  return AMJ_OK;
+  if (off[0] != 0 || off[n0] != n1) {
+    set_error(ctx, AMJ_ERR_ARG, "trie offsets must start at 0 and end at the child count");
+    return AMJ_ERR_ARG;
+  }
+  std::vector<uint32_t> off32(n0 + 1);
+  for (uint64_t j = 0; j < n0; ++j) {
+    if (off[j + 1] < off[j] || (j > 0 && d0[j] <= d0[j - 1])) {
+      set_error(ctx, AMJ_ERR_ARG, "trie level 0 must be strictly increasing with monotone offsets");
+      return AMJ_ERR_ARG;
+    }
+    for (uint64_t k = off[j] + 1; k < off[j + 1]; ++k)
+      if (d1[k] < d1[k - 1]) {
+        set_error(ctx, AMJ_ERR_ARG, "trie child lists must be sorted");
+        return AMJ_ERR_ARG;
+      }
+    off32[j] = (uint32_t)off[j];
+  }
+  off32[n0] = (uint32_t)n1;
+  hipStream_t s = ctx->stream;
+  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[0], s));
+  AMJ_TRY(dmalloc(ctx, (void**)&out->d0, n0 * 4));
+  out->n0 = (uint32_t)n0;
+  AMJ_TRY(dmalloc(ctx, (void**)&out->off, (n0 + 1) * 4));
+  AMJ_TRY(dmalloc(ctx, (void**)&out->d1, n1 * 4));
+  out->n1 = (uint32_t)n1;
+  if (n0) AMJ_HIP_TRY(ctx, hipMemcpyAsync(out->d0, d0, n0 * 4, hipMemcpyHostToDevice, s));
+  if (n1) AMJ_HIP_TRY(ctx, hipMemcpyAsync(out->d1, d1, n1 * 4, hipMemcpyHostToDevice, s));
+  AMJ_HIP_TRY(ctx, hipMemcpyAsync(out->off, off32.data(), (n0 + 1) * 4, hipMemcpyHostToDevice, s));
+  AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
+  return finish_trie(ctx, out);
 }
 
 void free_trie(amj_ctx_t* ctx, TrieDev* t) {

@@ -165,10 +165,45 @@ int alftj_trie_copy_level(amj_ctx_t* ctx, const amj_trie_t* trie, int level, int
  * materialises (opts->materialize) and checksums; ALFTJ_BFS returns count and checksum. */
 int alftj_run(amj_ctx_t* ctx, amj_trie_t* const* tries, uint32_t n, const uint32_t* attr_order,
               uint32_t n_attrs, int algo, const amj_opts_t* opts, amj_result_t* res);
+/* As alftj_run, with the attributes of trie i given as trie_attrs[2i], trie_attrs[2i+1] instead
+ * of the trie's own: one trie can then appear several times under different attributes, as
+ * SELF mode aliases Trie[0] (exec_LFTJ.cu:198-222). */
+int alftj_run_aliased(amj_ctx_t* ctx, amj_trie_t* const* tries, uint32_t n,
+                      const uint32_t* trie_attrs, const uint32_t* attr_order, uint32_t n_attrs,
+                      int algo, const amj_opts_t* opts, amj_result_t* res);
 /* exec-LFTJ SELF mode (exec_LFTJ.cu:198-231): one trie over `edges`, aliased as R(0,1),
  * S(1,2), T(0,2) for TRI; FOUR adds (2,3), (0,3), (1,3) (the 4-clique of exec_AMHJ.cu:59). */
 int alftj_pattern(amj_ctx_t* ctx, const amj_rel_t* edges, int query, int algo,
                   const amj_opts_t* opts, amj_result_t* res);
+/* A trie from host CSR arrays (IndexedTrie data/offsets of a 2-level trie,
+ * IndexedTrie.cuh:21-54): d0[n0] strictly increasing, off[n0+1] (off[0] = 0, off[n0] = n1),
+ * d1[n1] sorted within each run. Duplicate children are kept. attr_ids may be NULL ({0,1}). */
+int alftj_trie_from_csr(amj_ctx_t* ctx, uint64_t n0, const int32_t* d0, const uint64_t* off,
+                        uint64_t n1, const int32_t* d1, const uint32_t* attr_ids,
+                        amj_trie_t** out);
+/* .tr files (IndexedTrie::serialization / init_with_deserialization, IndexedTrie.cuh:122-197):
+ * u32 num_attrs, u64 data_len[], u32 attr_list[], int32 levels, u64 offsets. Two levels only. */
+int alftj_trie_save(amj_ctx_t* ctx, const amj_trie_t* trie, const char* path);
+int alftj_trie_load(amj_ctx_t* ctx, const char* path, amj_trie_t** out);
+
+/* ---- files and result fingerprint (host side, no context needed) ----------------------- */
+/* .db relation files (read_rel_cols_mmap, common-utils/file_io.h:90-121, names as in
+ * QueryProcessor.cuh:41-73): <rows>_<attr0>_<attr1>.db holds column 0 then column 1, raw
+ * little-endian int32. parse_name fills rows, num_cols and up to max_cols attribute ids. */
+int amj_db_parse_name(const char* path, uint64_t* rows, uint32_t* num_cols, uint32_t* attr_ids,
+                      uint32_t max_cols);
+int amj_db_read(const char* path, uint64_t rows, uint32_t num_cols, int32_t* const* cols);
+/* Write a host relation as <dir>/<rows>_<attr0>_<attr1>.db; the path goes to path_out. */
+int amj_db_write(const char* dir, const amj_rel_t* rel, char* path_out, size_t path_len);
+/* Message of the last failed call that took no context (file functions). */
+const char* amj_file_error(void);
+/* res_stat (common-utils/res_stat.h:32-133): MD5 of the text "[h0, h1, ...]\n" of the histogram
+ * of (v & 0xFFFFF) over all values of all columns, up to the largest such value. hex_out gets
+ * 32 hex digits and a NUL. amj_result_md5 histograms a materialised result on the device;
+ * amj_res_stat_md5 takes host columns. */
+int amj_result_md5(amj_ctx_t* ctx, const amj_result_t* res, char* hex_out);
+int amj_res_stat_md5(const int32_t* const* cols, uint32_t num_cols, uint64_t rows,
+                     char* hex_out);
 
 #ifdef __cplusplus
 }

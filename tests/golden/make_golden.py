@@ -1,9 +1,13 @@
 """Writes tests/golden/mhj_cpu_counts.json: join counts printed by the reference's own CPU
 multi-way hash join (MHJ-CPU/MHJ.cpp, built by `make -C oracle ref` into oracle/_ref/exec-MHJ)
-on chain datasets produced by this package's deterministic generators.
+on chain datasets produced by this package's deterministic generators; and
+tests/golden/res_stat_md5.json: the reference's result fingerprint (res_stat,
+common-utils/res_stat.h, built with a small driver into oracle/_ref/res_stat_ref) of columns
+from the same generators.
 
-Run in the build container only (needs the reference sources to build the binary):
-    python tests/golden/make_golden.py
+Run in the build container only (needs the reference sources to build the binaries):
+    python tests/golden/make_golden.py             # join counts
+    python tests/golden/make_golden.py --res-stat  # result fingerprints
 
 MHJ-CPU takes a directory of `<rows>_<attr0>_<attr1>.db` files (raw int32 columns, column 0
 then column 1; common-utils/file_io.h:123-152) and uses readdir order as the table order
@@ -81,7 +85,49 @@ def run_mhj(d, threads=8):
     return int(m.group(1))
 
 
+RES_STAT_EXE = os.path.join(ROOT, "oracle", "_ref", "res_stat_ref")
+RES_STAT_OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "res_stat_md5.json")
+# rows, columns, value range [lo, hi), seed
+RES_STAT_CASES = [
+    (0, 2, 0, 10, 1), (1, 1, 5, 6, 1), (7, 3, -5, 5, 2), (1000, 2, 0, 100, 3),
+    (100000, 4, -(1 << 31), (1 << 31) - 1, 4), (50000, 6, 0, 1 << 20, 5),
+    (3000, 2, 1 << 20, (1 << 20) + 50, 6), (20000, 1, -1000, 0, 7),
+]
+
+
+def res_stat_columns(rows, ncols, lo, hi, seed):
+    return [G.uniform(rows, lo, hi, seed, c) for c in range(ncols)]
+
+
+def make_res_stat_golden():
+    if not os.path.exists(RES_STAT_EXE):
+        subprocess.check_call(["make", "-C", os.path.join(ROOT, "oracle"), "ref"])
+    out = {"generator": "accmultiwayjoins_amd.generators.uniform(rows, lo, hi, seed, column)",
+           "cases": []}
+    with tempfile.TemporaryDirectory() as d:
+        for rows, ncols, lo, hi, seed in RES_STAT_CASES:
+            path = os.path.join(d, "cols.bin")
+            with open(path, "wb") as f:
+                for c in res_stat_columns(rows, ncols, lo, hi, seed):
+                    f.write(np.ascontiguousarray(c, dtype="<i4").tobytes())
+            p = subprocess.run([RES_STAT_EXE, path, str(rows), str(ncols)], capture_output=True,
+                               text=True, timeout=600)
+            m = re.search(r"Md5sum of result histogram: ([0-9a-f]{32})", p.stdout + p.stderr)
+            if p.returncode != 0 or not m:
+                raise RuntimeError("res_stat_ref failed:\n" + (p.stdout + p.stderr)[-2000:])
+            rec = {"rows": rows, "cols": ncols, "lo": lo, "hi": hi, "seed": seed,
+                   "md5": m.group(1)}
+            print(rec, flush=True)
+            out["cases"].append(rec)
+    with open(RES_STAT_OUT, "w") as f:
+        json.dump(out, f, indent=1)
+    print("wrote", RES_STAT_OUT)
+
+
 def main():
+    if "--res-stat" in sys.argv:
+        make_res_stat_golden()
+        return
     if not os.path.exists(EXE):
         subprocess.check_call(["make", "-C", os.path.join(ROOT, "oracle"), "ref"])
     base = "/dev/shm" if os.path.isdir("/dev/shm") else None

@@ -107,3 +107,14 @@ def test_rmat_orient_properties():
     assert len(und) == len(s)  # each undirected edge once
     deg = np.bincount(np.concatenate([s, d]), minlength=1 << 10)
     assert np.all((deg[s] < deg[d]) | ((deg[s] == deg[d]) & (s < d)))
+
+
+def test_cli_usage_errors_exit_cleanly():
+    import subprocess
+    bindir = os.path.join(os.path.dirname(N.LIB_PATH), "bin")
+    for exe, args in (("exec-AMHJ", []), ("exec-LFTJ", []), ("exec-AMHJ", ["BAD", "x", "0", "0", "0", "0"]),
+                      ("exec-LFTJ", ["/nonexistent", "1", "0", "0"])):
+        p = subprocess.run([os.path.join(bindir, exe)] + args, capture_output=True, text=True,
+                           timeout=60)
+        assert p.returncode == 1, (exe, args, p.stdout, p.stderr)
+        assert "ERROR" in p.stderr
