@@ -19,7 +19,7 @@ STATUS = {
     0: "AMJ_OK", -1: "AMJ_ERR_ARG", -2: "AMJ_ERR_HIP", -3: "AMJ_ERR_NOMEM",
     -4: "AMJ_ERR_NODEV", -5: "AMJ_ERR_STATE",
 }
-PATHS = {1: "plain", 2: "fib", 3: "ws", 4: "dro", 5: "lftj-dfs", 6: "lftj-bfs"}
+PATHS = {1: "plain", 2: "fib", 3: "ws", 4: "dro", 5: "lftj-dfs", 6: "lftj-bfs", 7: "ws-dro"}
 
 QUERY = {"NORMAL": 0, "TRI": 1, "FOUR": 2}
 ALGO = {"BFS": 0, "DFS": 1}
@@ -66,7 +66,9 @@ class AmjResult(C.Structure):
         ("probe_kernel_ms", C.c_double),
         ("probe_kernel_launches", C.c_uint64),
         ("path", C.c_uint32),
-        ("reserved", C.c_uint32 * 5),
+        ("reserved", C.c_uint32 * 3),
+        ("detail", C.c_uint32),
+        ("passes", C.c_uint32),
     ]
 
 
@@ -99,6 +101,7 @@ SIGNATURES = [
     ("amj_result_copy_col", C.c_int,
      [C.c_void_p, C.POINTER(AmjResult), C.c_uint32, C.c_uint64, C.c_uint64, C.c_void_p]),
     ("alftj_trie_build", C.c_int, [C.c_void_p, C.POINTER(AmjRel), C.POINTER(C.c_void_p)]),
+    ("alftj_trie_build_resident", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
     ("alftj_trie_destroy", None, [C.c_void_p, C.c_void_p]),
     ("alftj_trie_info", C.c_int,
      [C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),

@@ -39,6 +39,8 @@ class JoinResult:
     probe_kernel_ms: float
     cols: Optional[list] = None  # numpy int32 columns when fetched
     rows_written: int = 0
+    detail: int = 0   # fib: suffix-table depth; ws: tasks shared between waves
+    passes: int = 0   # probe traversals run
     _res: Optional[N.AmjResult] = field(default=None, repr=False)
 
     def tuples(self) -> np.ndarray:
@@ -219,7 +221,7 @@ class Context:
             attr_ids=[res.attr_ids[i] for i in range(res.num_cols)],
             path=N.PATHS.get(res.path, str(res.path)), build_ms=res.build_ms,
             probe_ms=res.probe_ms, probe_kernel_ms=res.probe_kernel_ms,
-            rows_written=res.rows_written, _res=res)
+            rows_written=res.rows_written, detail=res.detail, passes=res.passes, _res=res)
         if fetch and res.rows_written:
             cols = []
             for c in range(res.num_cols):
@@ -261,6 +263,12 @@ class Context:
 
     def trie(self, cols, attr_ids=(0, 1)) -> Trie:
         return Trie(self, cols, attr_ids)
+
+    def trie_from_relation(self, rel: Relation, attr_ids=(0, 1)) -> Trie:
+        """A trie over a device-resident two-column relation."""
+        h = C.c_void_p()
+        self._check(N.lib().alftj_trie_build_resident(self.h, rel.h, C.byref(h)))
+        return Trie(self, attr_ids=tuple(attr_ids), handle=h)
 
     def trie_from_csr(self, d0, off, d1, attr_ids=(0, 1)) -> Trie:
         """A trie from CSR arrays (level-0 values, u64 offsets, children)."""

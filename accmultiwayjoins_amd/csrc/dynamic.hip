@@ -1,0 +1,465 @@
+// Persistent AMHJ probe with work sharing (ws) and direct result output (dro).
+//
+// Reference: AMHJ_WS (MHJ-GPU/Joins/AMHJ_WS.cuh:32-557) keeps persistent warps that claim probe
+// tuples with an atomic counter and, when a warp meets a long bucket, appends the tail as
+// 1024-entry tasks to a TaskBook (TaskBook.cuh:13-109) served through a global queue
+// (GCQueue.cuh:30-107) until an idle-warp barrier (CBarrier.cuh:9-46) sees every warp idle.
+// AMHJ_DRO (Joins/AMHJ_DRO.cuh:18-707) drops the count pass: it sizes the output from a sample
+// and writes into chunks in a single pass, falling back to residual passes when chunks run out.
+//
+// Here persistent waves run the frontier DFS of probe_engine.h:
+//   * a wave claims units of 64 probe rows from a global counter;
+//   * when a level of its frontier has more than `split` children left, it keeps the next
+//     `split` and publishes the rest as tasks of `split` children each; a task is the 64-lane
+//     frontier of that level (bindings, runs, prefix) plus its child range, so the wave that
+//     takes it continues exactly where the owner would have;
+//   * results are placed with one atomic reservation per last-level frontier (not per tuple);
+//   * a wave leaves when no unit is left, the queue is empty and no wave holds work (a wave
+//     holding work is running, so it finishes and, having pushed tasks, takes them itself if
+//     nobody else does). Every wait is bounded: a wave that sees no unit or task completed
+//     anywhere for the time limit raises an error flag and leaves, so the kernel always ends.
+// With dro the output is allocated from a sampled estimate before the single pass; if the
+// reservations overrun it, the pass knows the exact total and is run once more into an exact
+// allocation.
+#include "probe_engine.h"
+#include "runtime.h"
+
+namespace amj {
+
+// Control block of one launch (zeroed by the host).
+struct DynCtl {
+  unsigned long long next_unit;  // units claimed
+  unsigned long long q_tail;     // task slots reserved
+  unsigned long long q_head;     // task slots claimed
+  unsigned long long out_rows;   // output rows reserved (== results once the pass ends)
+  unsigned long long count;      // results
+  unsigned long long csum;       // checksum, sum of the waves' partials mod 2^64
+  unsigned long long pushed;     // tasks published
+  unsigned long long progress;   // units and tasks completed
+  unsigned int busy;             // waves holding a unit or a task
+  unsigned int flags;            // 1: a wait hit the time limit; 2: output capacity overrun
+};
+
+constexpr uint32_t kFlagSpin = 1u;
+constexpr uint32_t kFlagOverflow = 2u;
+
+// Task record: 16 header words then 10 words per lane (bindings, lo, len, excl).
+constexpr int kTaskHdr = 16;
+constexpr int kTaskLaneWords = kMaxAttrs + 4;
+constexpr int kTaskWords = kTaskHdr + kTaskLaneWords * kWave;
+
+struct DynArgs {
+  DynCtl* ctl;
+  uint32_t* tasks;   // cap * kTaskWords
+  uint32_t* ready;   // cap flags
+  uint64_t cap;      // task slots
+  uint64_t split;    // children per task (multiple of 64); 0 disables sharing
+  uint64_t units;    // units of 64 probe rows
+  uint64_t out_cap;  // output rows available
+  uint64_t stall_ticks;  // wall-clock ticks without any completed work before a wave gives up
+};
+
+__device__ __forceinline__ unsigned long long ld_acq(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned int ld_acq(const unsigned int* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+// polling reads: relaxed, so idle waves do not invalidate the caches of the working ones
+__device__ __forceinline__ unsigned long long ld_rlx(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T add_acq_rel(T* p, T v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <typename T>
+__device__ __forceinline__ T add_relaxed(T* p, T v) {
+  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Wave-uniform result of an atomic done by lane 0.
+__device__ __forceinline__ uint64_t lane0_u64(uint64_t v) {
+  return uniform(__shfl(v, 0, kWave));
+}
+
+// Publish the level frontier `l` with child range [c0, c1) into slot `slot`.
+template <int t>
+__device__ __forceinline__ void write_task(const DynArgs& A, uint64_t slot, const Lv& l,
+                                           uint64_t c0, uint64_t c1) {
+  uint32_t* rec = A.tasks + slot * (uint64_t)kTaskWords;
+  const int lane = lane_id();
+  uint32_t* lw = rec + kTaskHdr;
+  sfor<0, kMaxAttrs>([&](auto a) { lw[a * kWave + lane] = (uint32_t)l.b[a]; });
+  lw[(kMaxAttrs + 0) * kWave + lane] = l.lo;
+  lw[(kMaxAttrs + 1) * kWave + lane] = l.len;
+  lw[(kMaxAttrs + 2) * kWave + lane] = (uint32_t)l.excl;
+  lw[(kMaxAttrs + 3) * kWave + lane] = (uint32_t)(l.excl >> 32);
+  if (lane == 0) {
+    rec[0] = (uint32_t)t;
+    rec[1] = (uint32_t)c0;
+    rec[2] = (uint32_t)(c0 >> 32);
+    rec[3] = (uint32_t)c1;
+    rec[4] = (uint32_t)(c1 >> 32);
+    rec[5] = (uint32_t)l.T;
+    rec[6] = (uint32_t)(l.T >> 32);
+  }
+  __threadfence();  // the record is visible device-wide before its flag
+  if (lane == 0) __hip_atomic_store(&A.ready[slot], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void read_task(const DynArgs& A, uint64_t slot, int& t, uint64_t& c0,
+                                          uint64_t& c1, Lv& l) {
+  const uint32_t* rec = A.tasks + slot * (uint64_t)kTaskWords;
+  const int lane = lane_id();
+  t = (int)uniform(rec[0]);
+  c0 = uniform(((uint64_t)rec[2] << 32) | rec[1]);
+  c1 = uniform(((uint64_t)rec[4] << 32) | rec[3]);
+  l.T = uniform(((uint64_t)rec[6] << 32) | rec[5]);
+  const uint32_t* lw = rec + kTaskHdr;
+  sfor<0, kMaxAttrs>([&](auto a) { l.b[a] = (int32_t)lw[a * kWave + lane]; });
+  l.lo = lw[(kMaxAttrs + 0) * kWave + lane];
+  l.len = lw[(kMaxAttrs + 1) * kWave + lane];
+  l.excl = ((uint64_t)lw[(kMaxAttrs + 3) * kWave + lane] << 32) | lw[(kMaxAttrs + 2) * kWave + lane];
+}
+
+// Reserve up to k task slots; returns the first slot, the number granted in *got.
+__device__ __forceinline__ uint64_t reserve_tasks(const DynArgs& A, uint64_t k, uint64_t* got) {
+  uint64_t first = 0, n = 0;
+  if (lane_id() == 0) {
+    for (int attempt = 0; attempt < 64; ++attempt) {
+      unsigned long long tl = ld_rlx(&A.ctl->q_tail);
+      if (tl >= A.cap) break;
+      const unsigned long long room = A.cap - tl;
+      const unsigned long long want = k < room ? k : room;
+      if (__hip_atomic_compare_exchange_strong(&A.ctl->q_tail, &tl, tl + want, __ATOMIC_ACQ_REL,
+                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+        first = tl;
+        n = want;
+        break;
+      }
+    }
+  }
+  *got = lane0_u64(n);
+  return lane0_u64(first);
+}
+
+template <int L, int t, int MODE>
+__device__ __forceinline__ void dyn_descend(const Plan& P, const Lv& cur, uint64_t c0, uint64_t c1,
+                            const ProbeOut& out, const DynArgs& A, Acc& acc) {
+  if constexpr (t == L - 1) {
+    // the last level is never split: its children are the results, [0, T)
+    if constexpr (MODE == MODE_COUNT) {
+      acc.count += cur.T;
+    } else if constexpr (!mode_mat<MODE>()) {
+      emit<false, mode_hash<MODE>()>(P, t, cur, 0, out, acc);
+      acc.count += cur.T;
+    } else {
+      uint64_t base = 0;
+      if (lane_id() == 0) base = add_relaxed(&A.ctl->out_rows, (unsigned long long)cur.T);
+      base = lane0_u64(base);
+      if (base + cur.T <= A.out_cap) {
+        emit<mode_mat<MODE>(), mode_hash<MODE>()>(P, t, cur, base, out, acc);
+      } else if (lane_id() == 0) {
+        __hip_atomic_fetch_or(&A.ctl->flags, kFlagOverflow, __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_AGENT);
+      }
+      acc.count += cur.T;
+    }
+  } else {
+    for (uint64_t c = c0; c < c1; c += kWave) {
+      if (A.split && c1 - c > A.split + (uint64_t)kWave) {
+        // keep [c, c + split), publish the remaining pieces (the last ones if the queue is short)
+        const uint64_t keep_end = c + A.split;
+        const uint64_t k = (c1 - keep_end + A.split - 1) / A.split;
+        uint64_t got = 0;
+        const uint64_t slot = reserve_tasks(A, k, &got);
+        if (got) {
+          const uint64_t first_piece = k - got;
+          for (uint64_t j = 0; j < got; ++j) {
+            const uint64_t p0 = keep_end + (first_piece + j) * A.split;
+            const uint64_t p1 = p0 + A.split < c1 ? p0 + A.split : c1;
+            write_task<t>(A, slot + j, cur, p0, p1);
+          }
+          if (lane_id() == 0) add_relaxed(&A.ctl->pushed, (unsigned long long)got);
+          c1 = keep_end + first_piece * A.split;
+        }
+      }
+      Lv ch;
+      make_child<t>(P, cur, c, c1, ch);
+      if (ch.T) dyn_descend<L, t + 1, MODE>(P, ch, 0, ch.T, out, A, acc);
+    }
+  }
+}
+
+// Claim a published task; -1: none available, -2: its record never became ready.
+__device__ __forceinline__ int64_t try_pop(const DynArgs& A) {
+  int64_t r = -1;
+  if (lane_id() == 0) {
+    unsigned long long h = ld_rlx(&A.ctl->q_head);
+    const unsigned long long tl = ld_rlx(&A.ctl->q_tail);
+    if (h < tl && __hip_atomic_compare_exchange_strong(&A.ctl->q_head, &h, h + 1,
+                                                       __ATOMIC_ACQ_REL, __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT)) {
+      r = (int64_t)h;
+      const uint64_t t0 = wall_clock64();
+      while (__hip_atomic_load(&A.ready[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+        if (wall_clock64() - t0 > A.stall_ticks) {
+          __hip_atomic_fetch_or(&A.ctl->flags, kFlagSpin, __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT);
+          r = -2;
+          break;
+        }
+        __builtin_amdgcn_s_sleep(2);
+      }
+    }
+  }
+  r = (int64_t)lane0_u64((uint64_t)r);
+  if (r >= 0)  // every lane acquires the record (its flag is set by now)
+    (void)__hip_atomic_load(&A.ready[r], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+  return r;
+}
+
+template <int L, int MODE>
+__global__ __launch_bounds__(kBlock) void dyn_kernel(const Plan P, const ProbeOut out,
+                                                     const DynArgs A) {
+  Acc acc{};
+  uint64_t seen = ~0ull, since = 0;  // progress counter last seen, and when it last moved
+  for (;;) {
+    // busy counts waves holding claimed work; it only keeps idle waves around to help (a wave
+    // that leaves early costs parallelism, never work: a pusher always serves its own tasks)
+    uint64_t u = ~0ull;
+    if (lane_id() == 0 && ld_rlx(&A.ctl->next_unit) < A.units)
+      u = add_relaxed(&A.ctl->next_unit, 1ull);
+    u = lane0_u64(u);
+    if (u < A.units) {
+      if (lane_id() == 0) add_relaxed(&A.ctl->busy, 1u);
+      Lv l0;
+      make_level0(P, u * kWave, l0);
+      if (l0.T) dyn_descend<L, 0, MODE>(P, l0, 0, l0.T, out, A, acc);
+      if (lane_id() == 0) {
+        add_relaxed(&A.ctl->progress, 1ull);
+        add_acq_rel(&A.ctl->busy, ~0u);
+      }
+      continue;
+    }
+    if (A.split == 0) break;  // without sharing there are no tasks to wait for
+    const int64_t slot = try_pop(A);
+    if (slot >= 0) {
+      if (lane_id() == 0) add_relaxed(&A.ctl->busy, 1u);
+      int t = 0;
+      uint64_t c0 = 0, c1 = 0;
+      Lv l;
+      read_task(A, (uint64_t)slot, t, c0, c1, l);
+      sfor<0, L - 1>([&](auto tt) {
+        if ((int)tt == t) dyn_descend<L, tt, MODE>(P, l, c0, c1, out, A, acc);
+      });
+      if (lane_id() == 0) {
+        add_relaxed(&A.ctl->progress, 1ull);
+        add_acq_rel(&A.ctl->busy, ~0u);
+      }
+      continue;
+    }
+    if (slot == -2) break;
+    uint32_t done = 0;
+    if (lane_id() == 0)
+      done = (ld_acq(&A.ctl->busy) == 0u && ld_rlx(&A.ctl->q_head) >= ld_rlx(&A.ctl->q_tail))
+                 ? 1u : 0u;
+    if (uniform(__shfl(done, 0, kWave))) break;
+    // no work here while other waves hold some: give up only if nothing completes for long
+    const uint64_t prog = lane0_u64(lane_id() == 0 ? ld_rlx(&A.ctl->progress) : 0ull);
+    const uint64_t now = wall_clock64();
+    if (prog != seen) {
+      seen = prog;
+      since = now;
+    } else if (now - since > A.stall_ticks) {
+      if (lane_id() == 0)
+        __hip_atomic_fetch_or(&A.ctl->flags, kFlagSpin, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      break;
+    }
+    __builtin_amdgcn_s_sleep(64);
+  }
+  const uint64_t cs = wave_sum_u64(acc.csum);
+  if (lane_id() == 0) {
+    add_relaxed(&A.ctl->count, (unsigned long long)acc.count);
+    add_relaxed(&A.ctl->csum, (unsigned long long)cs);
+  }
+}
+
+// Results of `samples` units spread evenly over the probe relation (the output estimate).
+template <int L>
+__global__ __launch_bounds__(kBlock) void dyn_sample_kernel(const Plan P, uint64_t units,
+                                                            uint64_t samples,
+                                                            unsigned long long* sum) {
+  const uint64_t s = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+  if (s >= samples) return;  // no barriers in this kernel
+  const uint64_t u = s * units / samples;
+  Lv l0;
+  make_level0(P, u * kWave, l0);
+  Acc acc{};
+  const DynArgs none{};  // no sharing, no output
+  if (l0.T) dyn_descend<L, 0, MODE_COUNT>(P, l0, 0, l0.T, ProbeOut{}, none, acc);
+  if (lane_id() == 0) add_relaxed(sum, (unsigned long long)acc.count);
+}
+
+hipError_t launch_dyn(const Plan& P, const ProbeOut& out, bool hash, const void* args_v,
+                      unsigned grid, hipStream_t s) {
+  const DynArgs& A = *(const DynArgs*)args_v;
+  const int mode = out.cols[0] ? (hash ? MODE_WRITE_HASH : MODE_WRITE) : (hash ? MODE_HASH
+                                                                              : MODE_COUNT);
+#define LAUNCH_M(LV, M) \
+  hipLaunchKernelGGL((dyn_kernel<LV, M>), dim3(grid), dim3(kBlock), 0, s, P, out, A)
+#define LAUNCH(LV)                                                 \
+  if (mode == MODE_WRITE) LAUNCH_M(LV, MODE_WRITE);                \
+  else if (mode == MODE_WRITE_HASH) LAUNCH_M(LV, MODE_WRITE_HASH); \
+  else if (mode == MODE_HASH) LAUNCH_M(LV, MODE_HASH);             \
+  else LAUNCH_M(LV, MODE_COUNT)
+  AMJ_DISPATCH_L(P, LAUNCH)
+#undef LAUNCH
+#undef LAUNCH_M
+  return hipGetLastError();
+}
+
+hipError_t launch_dyn_sample(const Plan& P, uint64_t samples, unsigned long long* sum,
+                             hipStream_t s) {
+  const uint64_t units = probe_units(P);
+  if (units == 0 || samples == 0) return hipSuccess;
+  const unsigned grid = (unsigned)((samples + kWavesPerBlock - 1) / kWavesPerBlock);
+#define LAUNCH(LV)                                                                       \
+  hipLaunchKernelGGL(dyn_sample_kernel<LV>, dim3(grid), dim3(kBlock), 0, s, P, units, \
+                     samples, sum)
+  AMJ_DISPATCH_L(P, LAUNCH)
+#undef LAUNCH
+  return hipGetLastError();
+}
+
+int dyn_occupancy_grid(const Plan& P, int cus) {
+  int per_cu = 0;
+  // every instantiation has the same launch bounds; the 5-level write kernel is the largest
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dyn_kernel<kMaxLevels, MODE_WRITE_HASH>,
+                                                   kBlock, 0) != hipSuccess || per_cu <= 0)
+    per_cu = 2;
+  (void)P;
+  return per_cu * cus;
+}
+
+// ------------------------------------------------------------------------------------------
+// Host driver.
+// ------------------------------------------------------------------------------------------
+constexpr uint64_t kDefaultSplit = 4096;      // children per shared task
+constexpr uint64_t kTaskSlots = 1 << 14;      // 16384 tasks x 2.6 KB
+constexpr uint64_t kStallMs = 20000;          // no completed work for this long: give up
+
+static int dyn_pass(amj_ctx_t* ctx, const Plan& P, const ProbeOut& out, bool hash,
+                    DynArgs A, unsigned grid, DynCtl* host_ctl, double* kernel_ms) {
+  hipStream_t s = ctx->stream;
+  AMJ_HIP_TRY(ctx, hipMemsetAsync(A.ctl, 0, sizeof(DynCtl), s));
+  if (A.split) AMJ_HIP_TRY(ctx, hipMemsetAsync(A.ready, 0, A.cap * 4, s));
+  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
+  AMJ_HIP_TRY(ctx, launch_dyn(P, out, hash, &A, grid, s));
+  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
+  AMJ_HIP_TRY(ctx, hipMemcpyAsync(host_ctl, A.ctl, sizeof(DynCtl), hipMemcpyDeviceToHost, s));
+  AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
+  if (kernel_ms) *kernel_ms += elapsed_ms(ctx->ev[3], ctx->ev[4]);
+  if (host_ctl->flags & kFlagSpin) {
+    set_error(ctx, AMJ_ERR_STATE, "persistent probe: no work completed within the time limit");
+    return AMJ_ERR_STATE;
+  }
+  return AMJ_OK;
+}
+
+int run_dynamic(amj_ctx_t* ctx, Plan& P, const amj_opts_t* o, amj_result_t* res) {
+  hipStream_t s = ctx->stream;
+  const uint64_t units = probe_units(P);
+  DynArgs A{};
+  void *ctl = nullptr, *tasks = nullptr, *ready = nullptr;
+  AMJ_TRY(scratch(ctx, SLOT_DYN_CTL, 256, &ctl));
+  A.ctl = (DynCtl*)ctl;
+  A.units = units;
+  int clock_khz = 0;
+  if (hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, ctx->device) !=
+          hipSuccess || clock_khz <= 0)
+    clock_khz = 100000;
+  A.stall_ticks = kStallMs * (uint64_t)clock_khz;
+  if (o->ws) {
+    A.split = o->ws_split_threshold ? (uint64_t)(o->ws_split_threshold + 63) / 64 * 64
+                                    : kDefaultSplit;
+    A.cap = kTaskSlots;
+    AMJ_TRY(scratch(ctx, SLOT_DYN_TASKS, A.cap * kTaskWords * 4, &tasks));
+    AMJ_TRY(scratch(ctx, SLOT_DYN_READY, A.cap * 4, &ready));
+    A.tasks = (uint32_t*)tasks;
+    A.ready = (uint32_t*)ready;
+  }
+  const unsigned grid = (unsigned)std::max<uint64_t>(
+      1, std::min<uint64_t>((units + kWavesPerBlock - 1) / kWavesPerBlock,
+                            (uint64_t)dyn_occupancy_grid(P, ctx->props.multiProcessorCount)));
+  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[2], s));
+  DynCtl hc{};
+  double kms = 0.0;
+  uint64_t capacity = 0;
+  uint32_t passes = 0;
+  const bool mat = o->materialize != 0;
+  if (!o->dro || !mat) {
+    // count pass, then an exact allocation (WS without DRO, AMHJ_WS.cuh:478-557)
+    if (mat) {
+      A.out_cap = 0;
+      AMJ_TRY(dyn_pass(ctx, P, ProbeOut{}, false, A, grid, &hc, &kms));
+      ++passes;
+      capacity = hc.count;
+    }
+  } else {
+    // DRO: output sized from a sample of units (AMHJ_DRO.cuh:590-640)
+    const uint64_t samples = std::min<uint64_t>(units, 4096);
+    void* sum = nullptr;
+    AMJ_TRY(scratch(ctx, SLOT_SUM, 64, &sum));
+    AMJ_HIP_TRY(ctx, hipMemsetAsync(sum, 0, 8, s));
+    AMJ_HIP_TRY(ctx, launch_dyn_sample(P, samples, (unsigned long long*)sum, s));
+    uint64_t sampled = 0;
+    AMJ_HIP_TRY(ctx, hipMemcpyAsync(&sampled, sum, 8, hipMemcpyDeviceToHost, s));
+    AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
+    const double est = samples ? (double)sampled * (double)units / (double)samples : 0.0;
+    capacity = (uint64_t)(est * 1.25) + 65536;
+    if (o->dro_capacity_rows) capacity = o->dro_capacity_rows;
+  }
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    ProbeOut out{};
+    if (mat && capacity > 0) {
+      const uint64_t need = capacity * (uint64_t)P.n_attrs * 4;
+      if (need > budget_left(ctx, o)) {
+        set_error(ctx, AMJ_ERR_NOMEM, "result of " + std::to_string(capacity) +
+                                          " tuples does not fit the HBM budget (use ooc=1 or "
+                                          "materialize=0)");
+        return AMJ_ERR_NOMEM;
+      }
+      AMJ_TRY(alloc_output(ctx, capacity, P.n_attrs, res));
+      for (int a = 0; a < P.n_attrs; ++a) out.cols[a] = res->cols[a];
+      out.capacity = capacity;
+    }
+    A.out_cap = capacity;
+    const bool hash = o->checksum || !mat;
+    AMJ_TRY(dyn_pass(ctx, P, out, hash, A, grid, &hc, &kms));
+    ++passes;
+    if (!(hc.flags & kFlagOverflow)) break;
+    // the estimate was short: the pass counted every result, rerun into an exact allocation
+    amj_result_free(ctx, res);
+    capacity = hc.out_rows;
+  }
+  if (hc.flags & kFlagOverflow) {
+    set_error(ctx, AMJ_ERR_STATE, "direct output overran an exact allocation");
+    return AMJ_ERR_STATE;
+  }
+  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[5], s));
+  AMJ_HIP_TRY(ctx, hipEventSynchronize(ctx->ev[5]));
+  res->count = hc.count;
+  res->checksum = hc.csum;
+  res->rows_written = mat ? hc.count : 0;
+  res->probe_ms = elapsed_ms(ctx->ev[2], ctx->ev[5]);
+  res->probe_kernel_ms = kms;
+  res->probe_kernel_launches = passes;
+  res->passes = passes;
+  res->detail = (uint32_t)std::min<unsigned long long>(hc.pushed, 0xffffffffull);
+  res->path = o->dro && mat ? (o->ws ? AMJ_PATH_WS_DRO : AMJ_PATH_DRO) : AMJ_PATH_WS;
+  return AMJ_OK;
+}
+
+}  // namespace amj

@@ -194,7 +194,8 @@ int run_fib(amj_ctx_t* ctx, Plan& P, const amj_opts_t* o, amj_result_t* res) {
   res->probe_kernel_launches = 1;
   res->rows_written = out.cols[0] ? total : 0;
   res->path = AMJ_PATH_FIB;
-  res->reserved[3] = (uint32_t)k;  // suffix depth used
+  res->detail = (uint32_t)k;  // suffix depth used
+  res->passes = 1;
   return AMJ_OK;
 }
 

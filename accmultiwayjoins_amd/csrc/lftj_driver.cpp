@@ -168,6 +168,25 @@ using namespace amj;
 
 extern "C" {
 
+int alftj_trie_build_resident(amj_ctx_t* ctx, const amj_relation_t* R, amj_trie_t** out) {
+  if (!ctx || !R || !out) return AMJ_ERR_ARG;
+  *out = nullptr;
+  if (R->num_cols != 2) {
+    set_error(ctx, AMJ_ERR_ARG, "a trie is built from a two-column relation");
+    return AMJ_ERR_ARG;
+  }
+  std::unique_ptr<amj_trie> t(new amj_trie());
+  t->attr_ids[0] = R->attr_ids[0];
+  t->attr_ids[1] = R->attr_ids[1];
+  const int rc = build_trie(ctx, R->cols[0], R->cols[1], R->rows, &t->dev);
+  if (rc != AMJ_OK) {
+    free_trie(ctx, &t->dev);
+    return rc;
+  }
+  *out = t.release();
+  return AMJ_OK;
+}
+
 int alftj_trie_build(amj_ctx_t* ctx, const amj_rel_t* rel, amj_trie_t** out) {
   if (!ctx || !rel || !out) return AMJ_ERR_ARG;
   *out = nullptr;
@@ -177,17 +196,9 @@ int alftj_trie_build(amj_ctx_t* ctx, const amj_rel_t* rel, amj_trie_t** out) {
   }
   amj_relation_t* R = nullptr;
   AMJ_TRY(amj_relation_create(ctx, rel, &R));
-  std::unique_ptr<amj_trie> t(new amj_trie());
-  t->attr_ids[0] = R->attr_ids[0];
-  t->attr_ids[1] = R->attr_ids[1];
-  const int rc = build_trie(ctx, R->cols[0], R->cols[1], R->rows, &t->dev);
+  const int rc = alftj_trie_build_resident(ctx, R, out);
   amj_relation_destroy(ctx, R);
-  if (rc != AMJ_OK) {
-    free_trie(ctx, &t->dev);
-    return rc;
-  }
-  *out = t.release();
-  return AMJ_OK;
+  return rc;
 }
 
 void alftj_trie_destroy(amj_ctx_t* ctx, amj_trie_t* t) {

@@ -1,0 +1,113 @@
+// Frontier-DFS building blocks of the AMHJ probe shared by the unit engine (probe.hip) and the
+// persistent work-sharing / direct-output engine (dynamic.hip): the per-level frontier state,
+// level 0 from 64 probe rows, the next 64 children of a level, and the emission of a last
+// level's results.
+#pragma once
+
+#include "probe_common.h"
+
+namespace amj {
+
+// Frontier state of one level (per lane) and its wave-uniform total.
+struct Lv {
+  int32_t b[kMaxAttrs];  // bindings of this lane's partial tuple
+  uint32_t lo;           // run of matches in this level's table
+  uint32_t len;
+  uint64_t excl;         // exclusive prefix of len over the wave
+  uint64_t T;            // total children of the level (uniform)
+};
+
+struct Acc {
+  uint64_t count;     // results counted (uniform)
+  uint64_t out_base;  // absolute output row of the next result (uniform)
+  uint64_t csum;      // per-lane checksum partial
+};
+
+__device__ __forceinline__ void finish_level(Lv& l) {
+  if (__ballot(l.len >= (1u << 26)) == 0ull) {  // 64 runs sum below 2^32
+    const uint32_t incl = wave_scan_add_dpp(l.len);
+    l.excl = incl - l.len;
+    l.T = (uint64_t)lane63(incl);
+  } else {
+    const uint64_t incl = wave_incl_scan_u64((uint64_t)l.len);
+    l.excl = incl - l.len;
+    l.T = uniform(__shfl(incl, kWave - 1, kWave));
+  }
+}
+
+// Parent lane of flattened child g (g in [c, c+64)).
+__device__ __forceinline__ int parent_lane(const Lv& l, uint64_t c,
+                                           uint64_t gg) {
+  if ((l.T >> 32) == 0)
+    return (int)parent_of_slots((uint32_t)l.excl, l.len, (uint32_t)c);
+  return find_parent(l.excl, gg, l.T);
+}
+
+__device__ __forceinline__ void lookup_level(const Plan& P, int t, Lv& l, bool valid) {
+  const int32_t key = get_attr(l.b, P.key_attr[t]);
+  const bool check = P.val_mode[t] == VAL_CHECK;
+  const int32_t cval = check ? get_attr(l.b, P.val_attr[t]) : 0;
+  lookup_run(P.tab[t], key, check, cval, valid, l.lo, l.len);
+}
+
+__device__ __forceinline__ void make_level0(const Plan& P, uint64_t row0, Lv& l) {
+  const uint64_t r = row0 + (uint64_t)lane_id();
+  const bool valid = r < P.probe_n;
+  sfor<0, kMaxAttrs>([&](auto a) { l.b[a] = 0; });
+  for (int c = 0; c < P.probe_ncols; ++c)
+    set_attr(l.b, P.probe_attr[c], valid ? P.probe_cols[c][r] : 0);
+  lookup_level(P, 0, l, valid);
+  finish_level(l);
+}
+
+// Children [c, min(c+64, end)) of level t become the level-(t+1) frontier (end <= par.T).
+template <int t>
+__device__ __forceinline__ void make_child(const Plan& P, const Lv& par,
+                                           uint64_t c, uint64_t end, Lv& ch) {
+  const uint64_t g = c + (uint64_t)lane_id();
+  const bool valid = g < end;
+  const uint64_t gg = valid ? g : par.T - 1;
+  const int p = parent_lane(par, c, gg);
+  const uint32_t plo = __shfl(par.lo, p, kWave);
+  const uint32_t pex = (uint32_t)__shfl((uint32_t)par.excl, p, kWave);
+  const uint32_t row = plo + ((uint32_t)gg - pex);
+  sfor<0, kMaxAttrs>([&](auto a) { ch.b[a] = (a < P.n_attrs) ? __shfl(par.b[a], p, kWave) : 0; });
+  if (P.val_mode[t] == VAL_BIND) set_attr(ch.b, P.val_attr[t], P.tab[t].vals[row]);
+  lookup_level(P, t + 1, ch, valid);
+  finish_level(ch);
+}
+
+// Emit the results [0, T) of the last level at absolute rows out_row0 + e.
+template <bool MATERIALIZE, bool HASH>
+__device__ __forceinline__ void emit(const Plan& P, int t, const Lv& l,
+                                     uint64_t out_row0, const ProbeOut& out, Acc& acc) {
+  const bool bind = P.val_mode[t] == VAL_BIND;
+  const int vattr = P.val_attr[t];
+  const int32_t* vals = P.tab[t].vals;
+  const uint64_t e1 = l.T;
+  for (uint64_t c = 0; c < e1; c += kWave) {
+    const uint64_t g = c + (uint64_t)lane_id();
+    const bool valid = g < e1;
+    const uint64_t gg = valid ? g : e1 - 1;
+    const int p = parent_lane(l, c, gg);
+    const uint32_t plo = __shfl(l.lo, p, kWave);
+    const uint32_t pex = (uint32_t)__shfl((uint32_t)l.excl, p, kWave);
+    const uint32_t row = plo + ((uint32_t)gg - pex);
+    int32_t v[kMaxAttrs];
+    sfor<0, kMaxAttrs>([&](auto a) { v[a] = (a < P.n_attrs) ? __shfl(l.b[a], p, kWave) : 0; });
+    if (bind) set_attr(v, vattr, vals[row]);
+    if (valid) {
+      if (HASH) acc.csum += hash_tuple(v, P.n_attrs);
+      if (MATERIALIZE) {
+        const uint64_t pos = out_row0 + g;
+        if (pos < out.capacity) {
+          sfor<0, kMaxAttrs>([&](auto a) {
+            if (a < P.n_attrs) out.cols[a][pos] = v[a];
+          });
+        }
+      }
+    }
+  }
+}
+
+}  // namespace amj

@@ -349,6 +349,7 @@ static int run_plain(amj_ctx_t* ctx, Plan& P, const amj_opts_t* o, amj_result_t*
   res->probe_kernel_launches = 1;
   res->rows_written = out.cols[0] ? total : 0;
   res->path = AMJ_PATH_PLAIN;
+  res->passes = 2;
   return AMJ_OK;
 }
 
@@ -367,6 +368,7 @@ int execute_plan(amj_ctx_t* ctx, Plan& plan, const std::vector<uint32_t>& attr_i
     res->num_cols = (uint32_t)plan.n_attrs;
     for (int a = 0; a < plan.n_attrs; ++a) res->attr_ids[a] = attr_ids[a];
   }
+  if (o->ws || o->dro) return run_dynamic(ctx, plan, o, res);
   return run_plain(ctx, plan, o, res);
 }
 

@@ -75,6 +75,9 @@ enum ScratchSlot {
   SLOT_TRIE1,
   SLOT_TRIE2,
   SLOT_TRIE3,
+  SLOT_DYN_CTL,    // persistent probe: control block, task records, task flags
+  SLOT_DYN_TASKS,
+  SLOT_DYN_READY,
 };
 
 int dmalloc(amj_ctx_t* ctx, void** p, size_t bytes);
@@ -116,5 +119,7 @@ uint64_t budget_left(amj_ctx_t* ctx, const amj_opts_t* o);
 // Frequent-item path: suffix tables + output-window writer. AMJ_ERR_STATE asks the caller to
 // use the count/write path instead (a frontier level would exceed 2^32 children).
 int run_fib(amj_ctx_t* ctx, Plan& P, const amj_opts_t* o, amj_result_t* res);
+// Persistent probe with work sharing (o->ws) and/or direct result output (o->dro).
+int run_dynamic(amj_ctx_t* ctx, Plan& P, const amj_opts_t* o, amj_result_t* res);
 
 }  // namespace amj

@@ -7,7 +7,9 @@ columns a, e uniform over [0, 2^31). A step runs the three skews s = 0, 0.5, 1.0
 each join builds the three hash tables and probes with ws=1 dro=1 fib=1, materialising all
 ~2^32 result tuples (5 int32 columns) in HBM. Inputs are resident in HBM before timing.
 
-Prints one JSON line (see the keys below). --gpus N > 1 exits: this build runs on one GPU.
+A second leg times the triangle query (ALFTJ DFS, R-MAT scale 22) and reports it under
+"triangle". Prints one JSON line (see the keys below). --gpus N > 1 exits: this build runs on
+one GPU.
 """
 from __future__ import annotations
 
@@ -58,6 +60,55 @@ def cpu_baseline(rels, sample_rows, threads):
     return cnt, (pms + qms) / 1e3, wall
 
 
+def triangle_leg(ctx, G, scale, steps, warmup):
+    """Triangle query E(a,b) x E(b,c) x E(a,c) on an R-MAT graph (edge factor 16, parameters
+    0.57/0.19/0.19/0.05, symmetrised, deduplicated, degree-oriented): one step = trie build +
+    ALFTJ DFS count pass + write pass materialising every triangle (3 int32 columns)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    t0 = time.perf_counter()
+    src, dst = G.rmat_oriented(scale, 16, seed=42)
+    log(f"[bench] triangle: R-MAT scale {scale}, {len(src)} oriented edges "
+        f"({time.perf_counter() - t0:.1f} s)")
+    rel = ctx.relation([src, dst], [0, 1])  # edges resident in HBM before timing
+
+    tri = [(0, 1), (1, 2), (0, 2)]  # SELF mode: one trie aliased three times
+
+    def run(algo, **opts):
+        t = ctx.trie_from_relation(rel)
+        r = ctx.lftj([t, t, t], [0, 1, 2], algo, trie_attrs=tri, **opts)
+        r.build_ms = t.build_ms
+        t.close()
+        return r
+
+    def one():
+        r = run("DFS", materialize=True, checksum=False)
+        out = (r.count, r.build_ms, r.probe_ms, r.probe_kernel_ms)
+        ctx.free(r)
+        return out
+
+    for _ in range(warmup):
+        one()
+    N = __import__("accmultiwayjoins_amd._native", fromlist=["lib"])
+    N.lib().amj_sync(ctx.h)
+    t = time.perf_counter()
+    recs = [one() for _ in range(steps)]
+    N.lib().amj_sync(ctx.h)
+    el = time.perf_counter() - t
+    cnt = recs[-1][0]
+    chk = run("DFS", materialize=False)
+    bfs = run("BFS", materialize=False)
+    if (chk.count, chk.checksum) != (bfs.count, bfs.checksum) or chk.count != cnt:
+        raise SystemExit("bench: triangle DFS / BFS disagree")
+    rel.close()
+    return {"value": cnt * steps / el, "unit": "tuples/s", "count": cnt,
+            "checksum": "%#018x" % chk.checksum, "ms_per_step": el / steps * 1e3,
+            "trie_build_ms": round(recs[-1][1], 3), "probe_ms": round(recs[-1][2], 3),
+            "write_kernel_ms": round(recs[-1][3], 3), "bfs_ms": round(bfs.probe_ms, 3),
+            "workload": f"ALFTJ DFS triangle, R-MAT scale {scale} edge factor 16, "
+                        f"{len(src)} oriented edges, trie build + count + materialise",
+            "verified_against": "ALFTJ BFS (count + checksum)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,6 +123,8 @@ def main():
     ap.add_argument("--cpu-sample-rows", type=int, default=1 << 21)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-triangle", action="store_true")
+    ap.add_argument("--tri-scale", type=int, default=22)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
@@ -158,6 +211,10 @@ def main():
                          "verified_against": checked[s][3]}
                 for (s, cnt, cs, bms, pms, kms, path, _x) in last}
 
+    tri = None
+    if not args.no_triangle:
+        tri = triangle_leg(ctx, G, args.tri_scale, args.steps, args.warmup)
+
     cpu = None
     if not args.no_cpu:
         s0 = skews[-1]
@@ -191,6 +248,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "cpu_baseline": cpu,
+        "triangle": tri,
     }
     ctx.close()
     print(json.dumps(out), flush=True)

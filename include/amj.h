@@ -95,7 +95,9 @@ typedef struct {
   double probe_kernel_ms;       /* time of the dominant probe kernel only */
   uint64_t probe_kernel_launches;
   uint32_t path;                /* which engine ran (AMJ_PATH_*) */
-  uint32_t reserved[5];
+  uint32_t reserved[3];         /* library bookkeeping of the result columns */
+  uint32_t detail;              /* fib: suffix-table depth; ws: tasks shared between waves */
+  uint32_t passes;              /* probe traversals run (dro: 1, or 2 after an overflow) */
 } amj_result_t;
 
 #define AMJ_PATH_PLAIN 1        /* count pass + scan + write pass */
@@ -104,6 +106,7 @@ typedef struct {
 #define AMJ_PATH_DRO 4
 #define AMJ_PATH_LFTJ_DFS 5
 #define AMJ_PATH_LFTJ_BFS 6
+#define AMJ_PATH_WS_DRO 7
 
 /* ---- context ------------------------------------------------------------------------ */
 int amj_ctx_create(int device, amj_ctx_t** out);
@@ -151,6 +154,8 @@ int amj_result_copy_col(amj_ctx_t* ctx, const amj_result_t* res, uint32_t col, u
  * level 0 = distinct col0 values, level 1 = their sorted distinct col1 children. Duplicate
  * pairs are dropped (the triejoin intersects sets). attr_ids of `rel` name the two levels. */
 int alftj_trie_build(amj_ctx_t* ctx, const amj_rel_t* rel, amj_trie_t** out);
+/* The same from a relation already resident on the device (amj_relation_create). */
+int alftj_trie_build_resident(amj_ctx_t* ctx, const amj_relation_t* rel, amj_trie_t** out);
 void alftj_trie_destroy(amj_ctx_t* ctx, amj_trie_t* trie);
 /* Level sizes, the longest child list and the build time of a trie. */
 int alftj_trie_info(amj_ctx_t* ctx, const amj_trie_t* trie, uint64_t* level0, uint64_t* children,

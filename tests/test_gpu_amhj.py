@@ -15,8 +15,11 @@ pytestmark = pytest.mark.gpu
 
 GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "mhj_cpu_counts.json")))
 
-PATHS = [dict(), dict(fib=1), dict(fib=1, fib_suffix=1), dict(fib=1, fib_suffix=2)]
-PATH_IDS = ["plain", "fib", "fib-nosuffix", "fib-deepsuffix"]
+PATHS = [dict(), dict(fib=1), dict(fib=1, fib_suffix=1), dict(fib=1, fib_suffix=2),
+         dict(ws=1), dict(ws=1, ws_split_threshold=64), dict(dro=1),
+         dict(dro=1, dro_capacity_rows=64), dict(ws=1, dro=1, ws_split_threshold=128)]
+PATH_IDS = ["plain", "fib", "fib-nosuffix", "fib-deepsuffix", "ws", "ws-split64", "dro",
+            "dro-overflow", "ws-dro"]
 
 
 def _rels(rec):
@@ -151,3 +154,34 @@ def test_errors_are_reported_not_fatal(ctx):
     with pytest.raises(AmjError) as e:
         ctx.chain_join(rels)
     assert "not bound" in str(e.value)
+
+
+@pytest.mark.parametrize("opts", [dict(ws=1), dict(ws=1, ws_split_threshold=64), dict(dro=1),
+                                  dict(ws=1, dro=1, dro_capacity_rows=1000,
+                                       ws_split_threshold=64)],
+                         ids=["ws", "ws-split64", "dro", "ws-dro-overflow"])
+def test_dynamic_paths_on_skew_and_patterns(ctx, opts):
+    # one key of the probe side hits a long run in every table: tasks get shared
+    n = 4096
+    a = np.zeros(n, np.int32)
+    a[::7] = np.arange(len(a[::7])) % 50
+    rels = [([np.arange(n, dtype=np.int32), a], [0, 1]),
+            ([np.zeros(3000, np.int32), np.arange(3000, dtype=np.int32) % 40], [1, 2]),
+            ([np.arange(3000, dtype=np.int32) % 40, np.arange(3000, dtype=np.int32)], [2, 3])]
+    want_cnt, want_cs, _, _, _ = O.mhj(rels)
+    r = ctx.chain_join(rels, materialize=True, **opts)
+    assert (r.count, r.checksum) == (want_cnt, want_cs)
+    assert r.path in ("ws", "dro", "ws-dro")
+    if opts.get("ws_split_threshold") == 64:
+        assert r.detail > 0  # tasks were shared
+    if opts.get("dro_capacity_rows"):
+        assert r.passes == 2  # the estimate was overrun and the pass ran again
+    ctx.free(r)
+    s, d = G.rmat_oriented(10, 16, seed=42)
+    n3, c3 = O.triangles(s, d)
+    n4, c4 = O.four_cliques(s, d)
+    r3 = ctx.pattern_join(s, d, "TRI", materialize=True, **opts)
+    assert (r3.count, r3.checksum) == (n3, c3)
+    ctx.free(r3)
+    r4 = ctx.pattern_join(s, d, "FOUR", materialize=False, **opts)
+    assert (r4.count, r4.checksum) == (n4, c4)
