@@ -4,6 +4,6 @@ The compute path is libamj.so (hand-written gfx950 HIP kernels behind a C ABI, s
 include/amj.h); this package is its host-side interface.
 """
 from .amj import AmjError, Context, HashTable, JoinResult, Relation, Trie  # noqa: F401
-from . import formats, generators  # noqa: F401
+from . import formats, generators, serving  # noqa: F401
 
-__all__ = ["AmjError", "Context", "HashTable", "JoinResult", "Relation", "Trie", "formats", "generators"]
+__all__ = ["AmjError", "Context", "HashTable", "JoinResult", "Relation", "Trie", "formats", "generators", "serving"]

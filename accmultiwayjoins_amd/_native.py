@@ -19,7 +19,7 @@ STATUS = {
     0: "AMJ_OK", -1: "AMJ_ERR_ARG", -2: "AMJ_ERR_HIP", -3: "AMJ_ERR_NOMEM",
     -4: "AMJ_ERR_NODEV", -5: "AMJ_ERR_STATE",
 }
-PATHS = {1: "plain", 2: "fib", 3: "ws", 4: "dro", 5: "lftj-dfs", 6: "lftj-bfs", 7: "ws-dro"}
+PATHS = {1: "plain", 2: "fib", 3: "ws", 4: "dro", 5: "lftj-dfs", 6: "lftj-bfs", 7: "ws-dro", 8: "batch"}
 
 QUERY = {"NORMAL": 0, "TRI": 1, "FOUR": 2}
 ALGO = {"BFS": 0, "DFS": 1}
@@ -97,9 +97,40 @@ SIGNATURES = [
      [C.c_void_p, C.POINTER(AmjRel), C.c_uint32, C.POINTER(AmjOpts), C.POINTER(AmjResult)]),
     ("amj_pattern_join", C.c_int,
      [C.c_void_p, C.POINTER(AmjRel), C.c_int, C.POINTER(AmjOpts), C.POINTER(AmjResult)]),
+    ("amj_execute_pattern", C.c_int,
+     [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.POINTER(AmjOpts), C.POINTER(AmjResult)]),
     ("amj_result_free", C.c_int, [C.c_void_p, C.POINTER(AmjResult)]),
     ("amj_result_copy_col", C.c_int,
      [C.c_void_p, C.POINTER(AmjResult), C.c_uint32, C.c_uint64, C.c_uint64, C.c_void_p]),
+    ("amj_batch_create", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    ("amj_batch_destroy", None, [C.c_void_p]),
+    ("amj_batch_add", C.c_int,
+     [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(AmjOpts),
+      C.POINTER(C.c_uint32)]),
+    ("amj_batch_add_pattern", C.c_int,
+     [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.POINTER(AmjOpts), C.POINTER(C.c_uint32)]),
+    ("amj_batch_set_budget", C.c_int, [C.c_void_p, C.c_uint64]),
+    ("amj_batch_run", C.c_int, [C.c_void_p]),
+    ("amj_batch_result", C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(AmjResult)]),
+    ("amj_batch_info", C.c_int,
+     [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
+    ("amj_server_create", C.c_int,
+     [C.c_int, C.c_uint64, C.c_uint32, C.c_int, C.POINTER(C.c_void_p)]),
+    ("amj_server_destroy", None, [C.c_void_p]),
+    ("amj_server_last_error", C.c_char_p, [C.c_void_p]),
+    ("amj_server_add_relation", C.c_int, [C.c_void_p, C.POINTER(AmjRel), C.POINTER(C.c_uint32)]),
+    ("amj_server_submit", C.c_int,
+     [C.c_void_p, C.POINTER(C.c_uint32), C.c_uint32, C.c_int, C.POINTER(C.c_uint64)]),
+    ("amj_server_submit_pattern", C.c_int,
+     [C.c_void_p, C.c_uint32, C.c_int, C.c_int, C.POINTER(C.c_uint64)]),
+    ("amj_server_wait", C.c_int,
+     [C.c_void_p, C.c_uint64, C.POINTER(AmjResult), C.POINTER(C.c_double)]),
+    ("amj_server_release", C.c_int, [C.c_void_p, C.POINTER(AmjResult)]),
+    ("amj_server_copy_col", C.c_int,
+     [C.c_void_p, C.POINTER(AmjResult), C.c_uint32, C.c_uint64, C.c_uint64, C.c_void_p]),
+    ("amj_server_stats", C.c_int,
+     [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64),
+      C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
     ("alftj_trie_build", C.c_int, [C.c_void_p, C.POINTER(AmjRel), C.POINTER(C.c_void_p)]),
     ("alftj_trie_build_resident", C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p)]),
     ("alftj_trie_destroy", None, [C.c_void_p, C.c_void_p]),

@@ -328,6 +328,15 @@ class Context:
                                           N.ALGO[algo], C.byref(o), C.byref(res)))
         return self._finish(res, fetch)
 
+    def execute_pattern(self, edges: Relation, table: HashTable, query: str = "TRI",
+                        fetch: bool = False, **opts) -> JoinResult:
+        """TRI / FOUR over a resident edge relation and its table (key 0, sort_vals=True)."""
+        o = _opts(**opts)
+        res = N.AmjResult()
+        self._check(N.lib().amj_execute_pattern(self.h, edges.h, table.h, N.QUERY[query],
+                                                C.byref(o), C.byref(res)))
+        return self._finish(res, fetch)
+
     def relation(self, cols, attr_ids) -> Relation:
         return Relation(self, cols, attr_ids)
 

@@ -1,0 +1,139 @@
+// Many AMHJ queries in one launch (batched count pass, one scan, batched write pass).
+//
+// The reference runs one query per process and one kernel per pass (exec_AMHJ.cu); it has no
+// batching. A small query (a 10 K-row probe is 157 units of 64 rows, 40 workgroups) leaves most
+// of the 256 CUs idle and pays several host round trips. Here the units of every query of a
+// batch are numbered consecutively; one grid covers all of them, each wave finds its query by a
+// binary search over the queries' first units (wave-uniform, scalar loads of a read-only
+// descriptor array) and runs the frontier DFS of probe_engine.h with that query's plan. One
+// exclusive scan over all unit counts gives every unit its output row inside its own query
+// (minus the query's first scanned value), and per-query checksums are summed with one atomic
+// per wave (a sum mod 2^64 does not depend on the order).
+#include "probe_engine.h"
+
+namespace amj {
+
+struct BatchQ {
+  Plan P;
+  ProbeOut out;     // capacity 0: count and checksum only
+  uint64_t unit0;   // first unit of this query in the batch
+  uint64_t units;
+};
+
+__device__ __forceinline__ uint32_t query_of(const BatchQ* __restrict__ qs, uint32_t nq,
+                                             uint64_t g) {
+  uint32_t lo = 0, hi = nq;  // last q with unit0 <= g
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (qs[mid].unit0 <= g) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <int L>
+__global__ __launch_bounds__(kBlock) void batch_count_kernel(const BatchQ* __restrict__ qs,
+                                                             uint32_t nq, uint64_t unit_begin,
+                                                             uint64_t total,
+                                                             uint64_t* __restrict__ counts) {
+  const uint64_t w = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+  if (w >= total) return;  // no barriers in this kernel
+  const uint64_t g = unit_begin + w;
+  const uint32_t q = uniform(query_of(qs, nq, g));
+  const BatchQ& Q = qs[q];
+  Lv l0;
+  make_level0(Q.P, (g - Q.unit0) * kWave, l0);
+  Acc acc{};
+  if (l0.T) descend<L, 0, MODE_COUNT>(Q.P, l0, ProbeOut{}, acc);
+  if (lane_id() == 0) counts[g] = acc.count;
+}
+
+template <int L>
+__global__ __launch_bounds__(kBlock) void batch_write_kernel(const BatchQ* __restrict__ qs,
+                                                             uint32_t nq, uint64_t unit_begin,
+                                                             uint64_t total,
+                                                             const uint64_t* __restrict__ base,
+                                                             unsigned long long* __restrict__ csums) {
+  const uint64_t w = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+  if (w >= total) return;  // no barriers in this kernel
+  const uint64_t g = unit_begin + w;
+  const uint32_t q = uniform(query_of(qs, nq, g));
+  const BatchQ& Q = qs[q];
+  Lv l0;
+  make_level0(Q.P, (g - Q.unit0) * kWave, l0);
+  Acc acc{};
+  acc.out_base = uniform(base[g] - base[Q.unit0]);
+  if (l0.T) descend<L, 0, MODE_WRITE_HASH>(Q.P, l0, Q.out, acc);
+  const uint64_t cs = wave_sum_u64(acc.csum);
+  if (lane_id() == 0 && cs)
+    __hip_atomic_fetch_add(&csums[q], (unsigned long long)cs, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Per-query totals: counts[q] = base[unit0 + units] - base[unit0].
+__global__ void batch_totals_kernel(const BatchQ* __restrict__ qs, uint32_t nq,
+                                    const uint64_t* __restrict__ base,
+                                    uint64_t* __restrict__ totals) {
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < nq) totals[q] = base[qs[q].unit0 + qs[q].units] - base[qs[q].unit0];
+}
+
+static unsigned grid_units(uint64_t total) {
+  return (unsigned)((total + kWavesPerBlock - 1) / kWavesPerBlock);
+}
+
+hipError_t launch_batch_count(int L, const void* qs, uint32_t nq, uint64_t unit_begin,
+                              uint64_t total, uint64_t* counts, hipStream_t s) {
+  if (total == 0) return hipSuccess;
+  const BatchQ* q = (const BatchQ*)qs;
+  switch (L) {
+#define C(LV)                                                                              \
+  case LV:                                                                                 \
+    hipLaunchKernelGGL(batch_count_kernel<LV>, dim3(grid_units(total)), dim3(kBlock), 0, s, \
+                       q, nq, unit_begin, total, counts);                                  \
+    break;
+    C(1) C(2) C(3) C(4) C(5)
+#undef C
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_write(int L, const void* qs, uint32_t nq, uint64_t unit_begin,
+                              uint64_t total, const uint64_t* base, unsigned long long* csums,
+                              hipStream_t s) {
+  if (total == 0) return hipSuccess;
+  const BatchQ* q = (const BatchQ*)qs;
+  switch (L) {
+#define C(LV)                                                                              \
+  case LV:                                                                                 \
+    hipLaunchKernelGGL(batch_write_kernel<LV>, dim3(grid_units(total)), dim3(kBlock), 0, s, \
+                       q, nq, unit_begin, total, base, csums);                             \
+    break;
+    C(1) C(2) C(3) C(4) C(5)
+#undef C
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_totals(const void* qs, uint32_t nq, const uint64_t* base,
+                               uint64_t* totals, hipStream_t s) {
+  if (nq == 0) return hipSuccess;
+  hipLaunchKernelGGL(batch_totals_kernel, dim3((nq + 255) / 256), dim3(256), 0, s,
+                     (const BatchQ*)qs, nq, base, totals);
+  return hipGetLastError();
+}
+
+size_t batch_desc_bytes() { return sizeof(BatchQ); }
+
+void batch_fill_desc(void* dst, const Plan& P, const ProbeOut& out, uint64_t unit0,
+                     uint64_t units) {
+  BatchQ* q = (BatchQ*)dst;
+  q->P = P;
+  q->out = out;
+  q->unit0 = unit0;
+  q->units = units;
+}
+
+}  // namespace amj

@@ -5,6 +5,41 @@
 
 using namespace amj;
 
+namespace amj {
+
+int make_pattern_plan(amj_ctx_t* ctx, const amj_relation* R, const amj_hash_table* H, int query,
+                      Plan* plan, std::vector<uint32_t>* ids) {
+  // exec_AMHJ.cu:42 A(0,1),B(1,2),C(0,2) and :59 A(0,1),B(1,2),C(2,3),D(0,3),E(0,2),F(1,3)
+  static const uint32_t tri[][2] = {{0, 1}, {1, 2}, {0, 2}};
+  static const uint32_t four[][2] = {{0, 1}, {1, 2}, {2, 3}, {0, 3}, {0, 2}, {1, 3}};
+  const uint32_t(*q)[2] = query == AMJ_QUERY_TRI ? tri : four;
+  const uint32_t nq = query == AMJ_QUERY_TRI ? 3 : 6;
+  if (query != AMJ_QUERY_TRI && query != AMJ_QUERY_FOUR) {
+    set_error(ctx, AMJ_ERR_ARG, "pattern query must be TRI or FOUR");
+    return AMJ_ERR_ARG;
+  }
+  if (!R || !H || R->num_cols != 2 || H->rel != R || H->key_col != 0 || !H->vals_sorted) {
+    set_error(ctx, AMJ_ERR_ARG,
+              "TRI/FOUR need the edge relation's table on column 0 built with sort_vals=1");
+    return AMJ_ERR_ARG;
+  }
+  ProbeUse pu{};
+  pu.rel = R;
+  pu.attr_ids[0] = q[0][0];
+  pu.attr_ids[1] = q[0][1];
+  std::vector<TableUse> uses;
+  for (uint32_t i = 1; i < nq; ++i) {
+    TableUse u{};
+    u.ht = H;
+    u.key_attr_id = q[i][0];
+    u.val_attr_id = q[i][1];
+    uses.push_back(u);
+  }
+  return make_plan(ctx, pu, uses, plan, ids, nullptr);
+}
+
+}  // namespace amj
+
 extern "C" {
 
 const char* amj_version(void) { return "amj 0.1 (gfx950)"; }
@@ -202,18 +237,7 @@ int amj_pattern_join(amj_ctx_t* ctx, const amj_rel_t* edges, int query, const am
     return AMJ_ERR_ARG;
   }
   std::memset(res, 0, sizeof(*res));
-  // exec_AMHJ.cu:42 A(0,1),B(1,2),C(0,2) and :59 A(0,1),B(1,2),C(2,3),D(0,3),E(0,2),F(1,3)
-  static const uint32_t tri[][2] = {{0, 1}, {1, 2}, {0, 2}};
-  static const uint32_t four[][2] = {{0, 1}, {1, 2}, {2, 3}, {0, 3}, {0, 2}, {1, 3}};
-  const uint32_t(*q)[2] = nullptr;
-  uint32_t nq = 0;
-  if (query == AMJ_QUERY_TRI) {
-    q = tri;
-    nq = 3;
-  } else if (query == AMJ_QUERY_FOUR) {
-    q = four;
-    nq = 6;
-  } else {
+  if (query != AMJ_QUERY_TRI && query != AMJ_QUERY_FOUR) {
     set_error(ctx, AMJ_ERR_ARG, "pattern query must be TRI or FOUR");
     return AMJ_ERR_ARG;
   }
@@ -226,27 +250,25 @@ int amj_pattern_join(amj_ctx_t* ctx, const amj_rel_t* edges, int query, const am
   // one table on the first column, aliased at every level (QueryProcessor.cuh:112-169)
   if (rc == AMJ_OK) rc = amj_hash_table_build(ctx, R, 0, 1, &H);
   if (rc == AMJ_OK) {
-    ProbeUse pu{};
-    pu.rel = R;
-    pu.attr_ids[0] = q[0][0];
-    pu.attr_ids[1] = q[0][1];
-    std::vector<TableUse> uses;
-    for (uint32_t i = 1; i < nq; ++i) {
-      TableUse u{};
-      u.ht = H;
-      u.key_attr_id = q[i][0];
-      u.val_attr_id = q[i][1];
-      uses.push_back(u);
-    }
     Plan plan;
     std::vector<uint32_t> ids;
-    rc = make_plan(ctx, pu, uses, &plan, &ids, nullptr);
+    rc = make_pattern_plan(ctx, R, H, query, &plan, &ids);
     if (rc == AMJ_OK) rc = execute_plan(ctx, plan, ids, opts, res);
     if (rc == AMJ_OK) res->build_ms = H->build_ms;
   }
   amj_hash_table_destroy(ctx, H);
   amj_relation_destroy(ctx, R);
   return rc;
+}
+
+int amj_execute_pattern(amj_ctx_t* ctx, const amj_relation_t* edges, const amj_hash_table_t* ht,
+                        int query, const amj_opts_t* opts, amj_result_t* res) {
+  if (!ctx || !edges || !ht || !res) return AMJ_ERR_ARG;
+  std::memset(res, 0, sizeof(*res));
+  Plan plan;
+  std::vector<uint32_t> ids;
+  AMJ_TRY(make_pattern_plan(ctx, edges, ht, query, &plan, &ids));
+  return execute_plan(ctx, plan, ids, opts, res);
 }
 
 int amj_result_free(amj_ctx_t* ctx, amj_result_t* res) {
@@ -256,6 +278,16 @@ int amj_result_free(amj_ctx_t* ctx, amj_result_t* res) {
     b.p = res->cols[0];
     b.bytes = (size_t)res->reserved[0] | ((size_t)res->reserved[1] << 32);
     ctx->out_pool.push_back(b);
+  } else if (res->reserved[2] == 2) {  // one of the results of a batch slab
+    void* base = (void*)(uintptr_t)((uint64_t)res->reserved[0] | ((uint64_t)res->reserved[1] << 32));
+    auto it = ctx->slabs.find(base);
+    if (it != ctx->slabs.end() && --it->second.refs == 0) {
+      amj_ctx::Buf b;
+      b.p = base;
+      b.bytes = it->second.bytes;
+      ctx->out_pool.push_back(b);
+      ctx->slabs.erase(it);
+    }
   }
   std::memset(res->cols, 0, sizeof(res->cols));
   res->reserved[2] = 0;

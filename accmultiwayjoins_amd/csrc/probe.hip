@@ -27,26 +27,6 @@ namespace amj {
 // ------------------------------------------------------------------------------------------
 // Unit engine: count pass and write pass over one unit of 64 probe rows.
 // ------------------------------------------------------------------------------------------
-template <int L, int t, int MODE>
-__device__ __forceinline__ void descend(const Plan& P, const Lv& cur,
-                                        const ProbeOut& out, Acc& acc) {
-  if constexpr (t == L - 1) {
-    if constexpr (MODE == MODE_COUNT) {
-      acc.count += cur.T;
-    } else {
-      emit<mode_mat<MODE>(), mode_hash<MODE>()>(P, t, cur, acc.out_base, out, acc);
-      acc.out_base += cur.T;
-      acc.count += cur.T;
-    }
-  } else {
-    for (uint64_t c = 0; c < cur.T; c += kWave) {
-      Lv ch;
-      make_child<t>(P, cur, c, cur.T, ch);
-      if (ch.T) descend<L, t + 1, MODE>(P, ch, out, acc);
-    }
-  }
-}
-
 template <int L>
 __global__ __launch_bounds__(kBlock) void probe_count_kernel(const Plan P,
                                                              uint64_t* __restrict__ counts,

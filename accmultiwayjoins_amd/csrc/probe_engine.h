@@ -110,4 +110,25 @@ __device__ __forceinline__ void emit(const Plan& P, int t, const Lv& l,
   }
 }
 
+// Depth-first over the frontier levels of one unit (count, or write at acc.out_base).
+template <int L, int t, int MODE>
+__device__ __forceinline__ void descend(const Plan& P, const Lv& cur,
+                                        const ProbeOut& out, Acc& acc) {
+  if constexpr (t == L - 1) {
+    if constexpr (MODE == MODE_COUNT) {
+      acc.count += cur.T;
+    } else {
+      emit<mode_mat<MODE>(), mode_hash<MODE>()>(P, t, cur, acc.out_base, out, acc);
+      acc.out_base += cur.T;
+      acc.count += cur.T;
+    }
+  } else {
+    for (uint64_t c = 0; c < cur.T; c += kWave) {
+      Lv ch;
+      make_child<t>(P, cur, c, cur.T, ch);
+      if (ch.T) descend<L, t + 1, MODE>(P, ch, out, acc);
+    }
+  }
+}
+
 }  // namespace amj

@@ -27,6 +27,12 @@ struct amj_ctx {
   std::map<int, Buf> scratch;
   // Parked result columns, reused by the next result that fits.
   std::vector<Buf> out_pool;
+  // Result slabs shared by the results of one batch: base -> (bytes, live results).
+  struct Slab {
+    size_t bytes = 0;
+    uint32_t refs = 0;
+  };
+  std::map<void*, Slab> slabs;
 };
 
 struct amj_relation {
@@ -119,6 +125,9 @@ uint64_t budget_left(amj_ctx_t* ctx, const amj_opts_t* o);
 // Frequent-item path: suffix tables + output-window writer. AMJ_ERR_STATE asks the caller to
 // use the count/write path instead (a frontier level would exceed 2^32 children).
 int run_fib(amj_ctx_t* ctx, Plan& P, const amj_opts_t* o, amj_result_t* res);
+// Plan of exec-AMHJ TRI / FOUR over one edge relation and its table (built with sort_vals=1).
+int make_pattern_plan(amj_ctx_t* ctx, const amj_relation* edges, const amj_hash_table* ht,
+                      int query, Plan* plan, std::vector<uint32_t>* ids);
 // Persistent probe with work sharing (o->ws) and/or direct result output (o->dro).
 int run_dynamic(amj_ctx_t* ctx, Plan& P, const amj_opts_t* o, amj_result_t* res);
 

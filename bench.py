@@ -16,6 +16,7 @@ from __future__ import annotations
 import argparse
 import ctypes as C
 import json
+import math
 import os
 import sys
 import time
@@ -109,6 +110,84 @@ def triangle_leg(ctx, G, scale, steps, warmup):
             "verified_against": "ALFTJ BFS (count + checksum)"}
 
 
+def serving_leg(G, n_queries, max_rows, rate, seed=42):
+    """Mixed stream (60% chains of 3/4/5 relations, 20% TRI, 20% FOUR) over resident relations:
+    chain relations log-uniform in [10^4, max_rows] rows with Zipf(s ~ U[0, 1.2]) build-side
+    keys (key range = the build size, so a chain returns about as many tuples as its probe
+    relation), graphs R-MAT scale 12..16; Poisson arrivals at `rate` queries/s. Served with
+    continuous batching, then the same stream one query per launch (the plain path)."""
+    import random
+    import threading
+    from accmultiwayjoins_amd.serving import Server
+    rng = random.Random(seed)
+    chains = []
+    for i in range(24):
+        k = 3 + i % 3
+        sizes = [int(10 ** rng.uniform(4, math.log10(max_rows))) for _ in range(k)]
+        sizes[1:] = [max(1000, x // 4) for x in sizes[1:]]
+        keys = max(1, min(sizes[1:]))
+        chains.append(G.chain_relations(sizes, keys, seed=seed + i, zipf_s=rng.uniform(0, 1.2),
+                                        payload_range=1 << 30))
+    graphs = [G.rmat_oriented(sc, 16, seed=seed + sc) for sc in (12, 13, 14, 15, 16)]
+    stream = []
+    for q in range(n_queries):
+        u = rng.random()
+        if u < 0.6:
+            stream.append(("chain", rng.randrange(len(chains))))
+        else:
+            stream.append(("TRI" if u < 0.8 else "FOUR", rng.randrange(len(graphs))))
+    gaps = [rng.expovariate(rate) for _ in range(n_queries)]
+
+    def serve(plain):
+        with Server(0, max_batch=256, plain=plain) as srv:
+            cids = [[srv.add_relation(c, a) for c, a in ch] for ch in chains]
+            gids = [srv.add_relation([sd[0], sd[1]], (0, 1)) for sd in graphs]
+            tickets, lat, outs = [], [], []
+            done = threading.Event()
+
+            def waiter():
+                for i in range(n_queries):
+                    while len(tickets) <= i:
+                        time.sleep(0.0001)
+                    cnt, cs, ms, _ = srv.wait(tickets[i])
+                    lat.append(ms)
+                    outs.append((cnt, cs))
+                done.set()
+
+            th = threading.Thread(target=waiter)
+            th.start()
+            t0 = time.perf_counter()
+            due = t0
+            for (kind, k), gap in zip(stream, gaps):
+                due += gap
+                d = due - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                if kind == "chain":
+                    tickets.append(srv.submit(cids[k], materialize=True))
+                else:
+                    tickets.append(srv.submit_pattern(gids[k], kind, materialize=True))
+            done.wait()
+            el = time.perf_counter() - t0
+            th.join()
+            st = srv.stats()
+        lat.sort()
+        return {"queries_per_s": n_queries / el, "p50_ms": lat[len(lat) // 2],
+                "p99_ms": lat[min(len(lat) - 1, int(len(lat) * 0.99))],
+                "batches": st["batches"], "seconds": el}, outs
+
+    served, a = serve(False)
+    plain, b = serve(True)
+    if a != b:
+        raise SystemExit("bench: served and plain results differ")
+    return {"served": served, "plain": plain,
+            "speedup": served["queries_per_s"] / plain["queries_per_s"],
+            "workload": f"{n_queries} mixed queries (60% chains of 3/4/5 relations of 1e4.."
+                        f"{max_rows} rows, 20% TRI, 20% FOUR on R-MAT 12..16), Poisson "
+                        f"arrivals at {rate} queries/s, results materialised",
+            "verified": "every result (count + checksum) equal in the served and plain runs"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,6 +203,10 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-triangle", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serve-queries", type=int, default=512)
+    ap.add_argument("--serve-max-rows", type=int, default=2_000_000)
+    ap.add_argument("--serve-rate", type=float, default=4000.0)
     ap.add_argument("--tri-scale", type=int, default=22)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -249,8 +332,17 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},
         "cpu_baseline": cpu,
         "triangle": tri,
+        "serving": None,
     }
+    for R in resident.values():
+        for r in R:
+            r.close()
     ctx.close()
+    if not args.no_serving:  # its own server context, after the chain's memory is released
+        serving = serving_leg(G, args.serve_queries, args.serve_max_rows, args.serve_rate)
+        log(f"[bench] serving: {serving['served']['queries_per_s']:.0f} q/s served vs "
+            f"{serving['plain']['queries_per_s']:.0f} q/s plain")
+        out["serving"] = serving
     print(json.dumps(out), flush=True)
 
 

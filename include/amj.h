@@ -52,6 +52,7 @@ typedef struct amj_ctx amj_ctx_t;
 typedef struct amj_relation amj_relation_t;
 typedef struct amj_hash_table amj_hash_table_t;
 typedef struct amj_trie amj_trie_t;
+typedef struct amj_batch amj_batch_t;
 
 /* A relation in the reference's column layout (Relation<D,C>, MHJ-GPU/Relation.cuh:9-48):
  * data[attr][row] int32, plus the attribute id of each column (attr_list). */
@@ -107,6 +108,7 @@ typedef struct {
 #define AMJ_PATH_LFTJ_DFS 5
 #define AMJ_PATH_LFTJ_BFS 6
 #define AMJ_PATH_WS_DRO 7
+#define AMJ_PATH_BATCH 8        /* many queries per launch (amj_batch_run) */
 
 /* ---- context ------------------------------------------------------------------------ */
 int amj_ctx_create(int device, amj_ctx_t** out);
@@ -143,10 +145,68 @@ int amj_chain_join(amj_ctx_t* ctx, const amj_rel_t* rels, uint32_t n, const amj_
 /* exec-AMHJ TRI / FOUR over one two-column edge relation (exec_AMHJ.cu:32-69). */
 int amj_pattern_join(amj_ctx_t* ctx, const amj_rel_t* edges, int query, const amj_opts_t* opts,
                      amj_result_t* res);
+/* TRI / FOUR over a resident edge relation and its table (key column 0, sort_vals=1), the
+ * resident counterpart of amj_pattern_join (build_hash_single, QueryProcessor.cuh:112-169). */
+int amj_execute_pattern(amj_ctx_t* ctx, const amj_relation_t* edges, const amj_hash_table_t* ht,
+                        int query, const amj_opts_t* opts, amj_result_t* res);
 int amj_result_free(amj_ctx_t* ctx, amj_result_t* res);
 /* Copy rows [row0, row0+n) of result column `col` to host memory. */
 int amj_result_copy_col(amj_ctx_t* ctx, const amj_result_t* res, uint32_t col, uint64_t row0,
                         uint64_t n, int32_t* host_out);
+
+/* ---- batches: many AMHJ queries per launch --------------------------------------------- */
+/* No counterpart in the reference (one query per process, exec_AMHJ.cu). Queries over
+ * resident relations and hash tables are queued, then run together: one count launch and one
+ * write launch per group of equal depth, one scan, two host round trips per batch. Each result
+ * equals the one amj_execute gives for the same query (count, checksum, tuples in the same
+ * order). Results are taken with amj_batch_result and released with amj_result_free. */
+int amj_batch_create(amj_ctx_t* ctx, amj_batch_t** out);
+void amj_batch_destroy(amj_batch_t* batch);
+/* Queue a NORMAL query (the arguments of amj_execute); *id numbers it within the batch.
+ * Only opts->materialize is used (the batch engine is the count / write unit engine). */
+int amj_batch_add(amj_batch_t* batch, const amj_relation_t* probe, amj_hash_table_t* const* hts,
+                  uint32_t n_hts, const amj_opts_t* opts, uint32_t* id);
+/* Queue a TRI / FOUR query over an edge relation and its table (key column 0, sort_vals=1). */
+int amj_batch_add_pattern(amj_batch_t* batch, const amj_relation_t* edges,
+                          const amj_hash_table_t* ht, int query, const amj_opts_t* opts,
+                          uint32_t* id);
+/* Bytes the batch's materialised results may take (default: what the device has free);
+ * amj_batch_run returns AMJ_ERR_NOMEM, with nothing run, when they would take more. */
+int amj_batch_set_budget(amj_batch_t* batch, uint64_t bytes);
+int amj_batch_run(amj_batch_t* batch);
+int amj_batch_result(amj_batch_t* batch, uint32_t id, amj_result_t* res);
+/* Queries, device time of the run, and the bytes its materialised results take (also set when
+ * amj_batch_run returned AMJ_ERR_NOMEM). */
+int amj_batch_info(const amj_batch_t* batch, uint32_t* queries, double* run_ms,
+                   uint64_t* result_bytes);
+
+/* ---- serving ----------------------------------------------------------------------------- */
+/* No counterpart in the reference. A server owns a context and a worker thread: relations are
+ * registered once (resident, hashed on column 0 with value-sorted runs), requests from any host
+ * thread are served with continuous batching (each cycle runs, as one batch, every request that
+ * arrived while the previous batch was on the GPU, up to max_batch), and results are admitted
+ * against hbm_budget_bytes (bytes of results held by clients until amj_server_release; 0 = half
+ * the device). plain = 1 serves one query per launch instead (the comparison path). */
+typedef struct amj_server amj_server_t;
+int amj_server_create(int device, uint64_t hbm_budget_bytes, uint32_t max_batch, int plain,
+                      amj_server_t** out);
+void amj_server_destroy(amj_server_t* server);
+/* Message of the last failed call on this server (valid until the next call). */
+const char* amj_server_last_error(amj_server_t* server);
+int amj_server_add_relation(amj_server_t* server, const amj_rel_t* rel, uint32_t* rel_id);
+/* NORMAL chain over registered relations (rel_ids[0] probes, the others are hashed on their
+ * first column), or TRI / FOUR over a registered edge relation. Returns at once. */
+int amj_server_submit(amj_server_t* server, const uint32_t* rel_ids, uint32_t n, int materialize,
+                      uint64_t* ticket);
+int amj_server_submit_pattern(amj_server_t* server, uint32_t edge_rel, int query, int materialize,
+                              uint64_t* ticket);
+/* Block until the request is served; latency_ms = submission to completion. */
+int amj_server_wait(amj_server_t* server, uint64_t ticket, amj_result_t* res, double* latency_ms);
+int amj_server_release(amj_server_t* server, amj_result_t* res);
+int amj_server_copy_col(amj_server_t* server, const amj_result_t* res, uint32_t col,
+                        uint64_t row0, uint64_t n, int32_t* host_out);
+int amj_server_stats(amj_server_t* server, uint64_t* batches, uint64_t* served, uint64_t* splits,
+                     double* busy_ms, uint64_t* bytes_held);
 
 /* ---- ALFTJ (leapfrog triejoin) ---------------------------------------------------------- */
 /* Sorted two-level CSR trie of a two-column relation (constructSortedTrie<D,C,2>,
