@@ -19,9 +19,11 @@ STATUS = {
     0: "AMJ_OK", -1: "AMJ_ERR_ARG", -2: "AMJ_ERR_HIP", -3: "AMJ_ERR_NOMEM",
     -4: "AMJ_ERR_NODEV", -5: "AMJ_ERR_STATE",
 }
-PATHS = {1: "plain", 2: "fib", 3: "ws", 4: "dro", 5: "lftj-dfs", 6: "lftj-bfs", 7: "ws-dro", 8: "batch"}
+PATHS = {1: "plain", 2: "fib", 3: "ws", 4: "dro", 5: "lftj-dfs", 6: "lftj-bfs", 7: "ws-dro", 8: "batch",
+         9: "ooc"}
 
 QUERY = {"NORMAL": 0, "TRI": 1, "FOUR": 2}
+AMJ_REL_HOST = 2
 ALGO = {"BFS": 0, "DFS": 1}
 
 
@@ -80,6 +82,9 @@ SIGNATURES = [
     ("amj_ctx_destroy", None, [C.c_void_p]),
     ("amj_last_error", C.c_char_p, [C.c_void_p]),
     ("amj_sync", C.c_int, [C.c_void_p]),
+    ("amj_host_alloc", C.c_int, [C.c_size_t, C.POINTER(C.c_void_p)]),
+    ("amj_host_free", None, [C.c_void_p]),
+    ("amj_memory_info", C.c_int, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     ("amj_device_info", C.c_int,
      [C.c_void_p, C.c_char_p, C.c_size_t, C.POINTER(C.c_uint64), C.POINTER(C.c_int)]),
     ("amj_relation_create", C.c_int, [C.c_void_p, C.POINTER(AmjRel), C.POINTER(C.c_void_p)]),

@@ -69,10 +69,23 @@ def _opts(ooc=0, ws=0, dro=0, fib=0, materialize=True, checksum=True, hbm_budget
     return o
 
 
+def pinned_empty(n: int, dtype=np.int32) -> np.ndarray:
+    """A numpy array in pinned host memory (amj_host_alloc); freed with the array."""
+    import weakref
+    itemsize = np.dtype(dtype).itemsize
+    p = C.c_void_p()
+    if N.lib().amj_host_alloc(max(1, n * itemsize), C.byref(p)) != N.AMJ_OK:
+        raise MemoryError("amj_host_alloc(%d bytes) failed" % (n * itemsize))
+    buf = (C.c_char * max(1, n * itemsize)).from_address(p.value)
+    weakref.finalize(buf, N.lib().amj_host_free, p.value)
+    return np.frombuffer(buf, dtype=dtype, count=n)
+
+
 class _RelKeep:
     """Keeps the ctypes arrays behind an AmjRel alive."""
 
-    def __init__(self, cols: Sequence[np.ndarray], attr_ids: Optional[Sequence[int]]):
+    def __init__(self, cols: Sequence[np.ndarray], attr_ids: Optional[Sequence[int]],
+                 host: bool = False):
         self.arrays = [np.ascontiguousarray(c, dtype=np.int32) for c in cols]
         rows = len(self.arrays[0]) if self.arrays else 0
         if any(len(a) != rows for a in self.arrays):
@@ -86,17 +99,20 @@ class _RelKeep:
         self.rel.rows = rows
         self.rel.cols = C.cast(self.ptrs, C.POINTER(C.c_void_p))
         self.rel.attr_ids = C.cast(self.ids, C.POINTER(C.c_uint32)) if self.ids else None
-        self.rel.on_device = 0
+        self.rel.on_device = N.AMJ_REL_HOST if host else 0
 
 
 class Relation:
-    """A device-resident relation (Relation<D,C>, MHJ-GPU/Relation.cuh:9-48)."""
+    """A relation (Relation<D,C>, MHJ-GPU/Relation.cuh:9-48): copied to HBM, or with
+    host=True kept in host memory (pinned_empty arrays stream fastest) as an out-of-core
+    probe relation."""
 
-    def __init__(self, ctx: "Context", cols, attr_ids):
-        keep = _RelKeep(cols, attr_ids)
+    def __init__(self, ctx: "Context", cols, attr_ids, host: bool = False):
+        keep = _RelKeep(cols, attr_ids, host)
         h = C.c_void_p()
         ctx._check(N.lib().amj_relation_create(ctx.h, C.byref(keep.rel), C.byref(h)))
         self.ctx, self.h, self.rows = ctx, h, keep.rel.rows
+        self._keep = keep if host else None  # host columns must outlive the relation
 
     def close(self):
         if self.h:
@@ -337,8 +353,14 @@ class Context:
                                                 C.byref(o), C.byref(res)))
         return self._finish(res, fetch)
 
-    def relation(self, cols, attr_ids) -> Relation:
-        return Relation(self, cols, attr_ids)
+    def relation(self, cols, attr_ids, host: bool = False) -> Relation:
+        return Relation(self, cols, attr_ids, host)
+
+    def allocated_bytes(self) -> int:
+        """Device bytes held by this context (relations, tables, scratch, pooled results)."""
+        a, p = C.c_uint64(), C.c_uint64()
+        self._check(N.lib().amj_memory_info(self.h, C.byref(a), C.byref(p)))
+        return a.value
 
     def hash_table(self, rel: Relation, key_col: int = 0, sort_vals: bool = False) -> HashTable:
         return HashTable(self, rel, key_col, sort_vals)

@@ -85,7 +85,7 @@ struct Plan {
   int32_t n_levels;                   // build tables, >= 1
   int32_t n_attrs;                    // attributes in a result tuple
   int32_t probe_ncols;
-  int32_t pad0;
+  int32_t probe_on_host;              // probe columns are host memory: streamed in slices
   uint64_t probe_n;
   const int32_t* probe_cols[AMJ_MAX_PROBE_COLS];
   int32_t probe_attr[AMJ_MAX_PROBE_COLS];

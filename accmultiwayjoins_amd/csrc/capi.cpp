@@ -91,6 +91,27 @@ const char* amj_last_error(const amj_ctx_t* ctx) {
   return ctx ? ctx->last_error.c_str() : "no context";
 }
 
+int amj_host_alloc(size_t bytes, void** out) {
+  if (!out) return AMJ_ERR_ARG;
+  *out = nullptr;
+  if (hipHostMalloc(out, bytes ? bytes : 1, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    return AMJ_ERR_NOMEM;
+  }
+  return AMJ_OK;
+}
+
+void amj_host_free(void* p) {
+  if (p) (void)hipHostFree(p);
+}
+
+int amj_memory_info(amj_ctx_t* ctx, uint64_t* allocated, uint64_t* peak) {
+  if (!ctx) return AMJ_ERR_ARG;
+  if (allocated) *allocated = ctx->allocated;
+  if (peak) *peak = ctx->peak;
+  return AMJ_OK;
+}
+
 int amj_sync(amj_ctx_t* ctx) {
   if (!ctx) return AMJ_ERR_ARG;
   AMJ_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -118,6 +139,15 @@ int amj_relation_create(amj_ctx_t* ctx, const amj_rel_t* r, amj_relation_t** out
   std::unique_ptr<amj_relation> rel(new amj_relation());
   rel->num_cols = r->num_cols;
   rel->rows = r->rows;
+  if (r->on_device == AMJ_REL_HOST) {  // stays in (preferably pinned) host memory
+    for (uint32_t c = 0; c < r->num_cols; ++c) {
+      rel->attr_ids[c] = r->attr_ids ? r->attr_ids[c] : c;
+      rel->cols[c] = const_cast<int32_t*>(r->cols[c]);
+    }
+    rel->host = true;
+    *out = rel.release();
+    return AMJ_OK;
+  }
   for (uint32_t c = 0; c < r->num_cols; ++c) {
     rel->attr_ids[c] = r->attr_ids ? r->attr_ids[c] : c;
     if (r->on_device) {

@@ -462,4 +462,157 @@ int run_dynamic(amj_ctx_t* ctx, Plan& P, const amj_opts_t* o, amj_result_t* res)
   return AMJ_OK;
 }
 
+
+// ------------------------------------------------------------------------------------------
+// Out of core (ooc.cuh:166-306): the probe relation stays in host memory and is streamed in
+// slices through two device buffers. Slice i+1 is copied on the side stream while slice i is
+// probed on the main stream (events order the reuse of a buffer); the persistent engine of
+// this file probes each slice and keeps accumulating the count, the checksum and the output
+// reservations in its control block, so the whole stream is enqueued without a host round
+// trip. Materialising takes a counting stream first (exact allocation), then a writing one.
+// ------------------------------------------------------------------------------------------
+namespace {
+
+struct OocEvents {
+  hipEvent_t copied[2]{}, done[2]{};
+  ~OocEvents() {
+    for (int b = 0; b < 2; ++b) {
+      if (copied[b]) (void)hipEventDestroy(copied[b]);
+      if (done[b]) (void)hipEventDestroy(done[b]);
+    }
+  }
+};
+
+}  // namespace
+
+static int ooc_stream(amj_ctx_t* ctx, const Plan& P, const ProbeOut& out, bool hash, DynArgs A,
+                      uint64_t slice, int32_t* const* bufs, unsigned grid, bool reset_out,
+                      DynCtl* hc, uint64_t* n_slices) {
+  hipStream_t s = ctx->stream, side = ctx->side;
+  OocEvents ev;
+  for (int b = 0; b < 2; ++b) {
+    AMJ_HIP_TRY(ctx, hipEventCreateWithFlags(&ev.copied[b], hipEventDisableTiming));
+    AMJ_HIP_TRY(ctx, hipEventCreateWithFlags(&ev.done[b], hipEventDisableTiming));
+  }
+  if (reset_out) AMJ_HIP_TRY(ctx, hipMemsetAsync(A.ctl, 0, sizeof(DynCtl), s));
+  // the side stream must not overwrite a buffer before the main stream has caught up
+  AMJ_HIP_TRY(ctx, hipEventRecord(ev.done[0], s));
+  AMJ_HIP_TRY(ctx, hipEventRecord(ev.done[1], s));
+  const uint64_t n = P.probe_n;
+  uint64_t k = 0;
+  for (uint64_t r0 = 0; r0 < n; r0 += slice, ++k) {
+    const int b = (int)(k & 1);
+    const uint64_t rows = n - r0 < slice ? n - r0 : slice;
+    AMJ_HIP_TRY(ctx, hipStreamWaitEvent(side, ev.done[b], 0));
+    for (int c = 0; c < P.probe_ncols; ++c)
+      if (rows)
+        AMJ_HIP_TRY(ctx, hipMemcpyAsync(bufs[b] + (uint64_t)c * slice, P.probe_cols[c] + r0,
+                                        rows * 4, hipMemcpyDefault, side));
+    AMJ_HIP_TRY(ctx, hipEventRecord(ev.copied[b], side));
+    AMJ_HIP_TRY(ctx, hipStreamWaitEvent(s, ev.copied[b], 0));
+    // fresh unit and task counters for the slice; count, checksum and reservations carry on
+    AMJ_HIP_TRY(ctx, hipMemsetAsync(A.ctl, 0, 3 * sizeof(unsigned long long), s));
+    if (A.split) AMJ_HIP_TRY(ctx, hipMemsetAsync(A.ready, 0, A.cap * 4, s));
+    Plan Q = P;
+    Q.probe_on_host = 0;
+    Q.probe_n = rows;
+    for (int c = 0; c < P.probe_ncols; ++c) Q.probe_cols[c] = bufs[b] + (uint64_t)c * slice;
+    DynArgs Ai = A;
+    Ai.units = (rows + kWave - 1) / kWave;
+    if (Ai.units) AMJ_HIP_TRY(ctx, launch_dyn(Q, out, hash, &Ai, grid, s));
+    AMJ_HIP_TRY(ctx, hipEventRecord(ev.done[b], s));
+  }
+  AMJ_HIP_TRY(ctx, hipMemcpyAsync(hc, A.ctl, sizeof(DynCtl), hipMemcpyDeviceToHost, s));
+  AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
+  AMJ_HIP_TRY(ctx, hipStreamSynchronize(side));
+  *n_slices = k;
+  if (hc->flags & kFlagSpin) {
+    set_error(ctx, AMJ_ERR_STATE, "persistent probe: no work completed within the time limit");
+    return AMJ_ERR_STATE;
+  }
+  return AMJ_OK;
+}
+
+int run_ooc(amj_ctx_t* ctx, Plan& P, const amj_opts_t* o, amj_result_t* res) {
+  hipStream_t s = ctx->stream;
+  const int nc = P.probe_ncols;
+  DynArgs A{};
+  void *ctl = nullptr, *tasks = nullptr, *ready = nullptr;
+  AMJ_TRY(scratch(ctx, SLOT_DYN_CTL, 256, &ctl));
+  A.ctl = (DynCtl*)ctl;
+  int clock_khz = 0;
+  if (hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, ctx->device) !=
+          hipSuccess || clock_khz <= 0)
+    clock_khz = 100000;
+  A.stall_ticks = kStallMs * (uint64_t)clock_khz;
+  if (o->ws) {
+    A.split = o->ws_split_threshold ? (uint64_t)(o->ws_split_threshold + 63) / 64 * 64
+                                    : kDefaultSplit;
+    A.cap = kTaskSlots;
+    AMJ_TRY(scratch(ctx, SLOT_DYN_TASKS, A.cap * kTaskWords * 4, &tasks));
+    AMJ_TRY(scratch(ctx, SLOT_DYN_READY, A.cap * 4, &ready));
+    A.tasks = (uint32_t*)tasks;
+    A.ready = (uint32_t*)ready;
+  }
+  // slice: the option, else a quarter of the room left split over the two buffers
+  uint64_t slice = o->ooc_slice_rows;
+  if (!slice) {
+    const uint64_t room = budget_left(ctx, o);
+    slice = room / 4 / (2ull * nc * 4);
+    slice = std::min<uint64_t>(slice, 1ull << 26);
+  }
+  slice = std::max<uint64_t>((slice + kWave - 1) / kWave * kWave, kWave);
+  void* buf[2] = {nullptr, nullptr};
+  AMJ_TRY(scratch(ctx, SLOT_OOC_A, slice * nc * 4, &buf[0]));
+  AMJ_TRY(scratch(ctx, SLOT_OOC_B, slice * nc * 4, &buf[1]));
+  int32_t* bufs[2] = {(int32_t*)buf[0], (int32_t*)buf[1]};
+  const unsigned grid = (unsigned)std::max<uint64_t>(
+      1, std::min<uint64_t>((slice / kWave + kWavesPerBlock - 1) / kWavesPerBlock,
+                            (uint64_t)dyn_occupancy_grid(P, ctx->props.multiProcessorCount)));
+  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[2], s));
+  DynCtl hc{};
+  uint64_t n_slices = 0;
+  const bool mat = o->materialize != 0;
+  uint32_t passes = 0;
+  ProbeOut out{};
+  if (mat) {  // counting stream, then the exact allocation
+    A.out_cap = 0;
+    AMJ_TRY(ooc_stream(ctx, P, ProbeOut{}, false, A, slice, bufs, grid, true, &hc, &n_slices));
+    ++passes;
+    const uint64_t total = hc.count;
+    if (total > 0) {
+      const uint64_t need = total * (uint64_t)P.n_attrs * 4;
+      if (need > budget_left(ctx, o)) {
+        set_error(ctx, AMJ_ERR_NOMEM, "result of " + std::to_string(total) +
+                                          " tuples does not fit the HBM budget (use "
+                                          "materialize=0)");
+        return AMJ_ERR_NOMEM;
+      }
+      AMJ_TRY(alloc_output(ctx, total, P.n_attrs, res));
+      for (int a = 0; a < P.n_attrs; ++a) out.cols[a] = res->cols[a];
+      out.capacity = total;
+    }
+    A.out_cap = total;
+  }
+  const bool hash = o->checksum || !mat;
+  AMJ_TRY(ooc_stream(ctx, P, out, hash, A, slice, bufs, grid, true, &hc, &n_slices));
+  ++passes;
+  if (hc.flags & kFlagOverflow) {
+    set_error(ctx, AMJ_ERR_STATE, "out-of-core probe overran its exact allocation");
+    return AMJ_ERR_STATE;
+  }
+  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[5], s));
+  AMJ_HIP_TRY(ctx, hipEventSynchronize(ctx->ev[5]));
+  res->count = hc.count;
+  res->checksum = hash ? hc.csum : 0;
+  res->rows_written = out.cols[0] ? hc.count : 0;
+  res->probe_ms = elapsed_ms(ctx->ev[2], ctx->ev[5]);
+  res->probe_kernel_ms = 0.0;
+  res->probe_kernel_launches = n_slices * passes;
+  res->passes = passes;
+  res->detail = (uint32_t)std::min<uint64_t>(n_slices, 0xffffffffull);
+  res->path = AMJ_PATH_OOC;
+  return AMJ_OK;
+}
+
 }  // namespace amj

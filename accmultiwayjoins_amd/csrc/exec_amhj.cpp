@@ -111,6 +111,8 @@ int main(int argc, char** argv) {
   amj_ctx_t* ctx = open_ctx();
   std::vector<amj_rel_t> views;
   for (auto& r : rels) views.push_back(r.view());
+  // out of core: the probe relation stays in host memory and is streamed in slices
+  if (o.ooc && query == AMJ_QUERY_NORMAL) views[0].on_device = AMJ_REL_HOST;
   amj_result_t res;
   const int rc = query == AMJ_QUERY_NORMAL
                      ? amj_chain_join(ctx, views.data(), (uint32_t)views.size(), &o, &res)

@@ -171,8 +171,8 @@ extern "C" {
 int alftj_trie_build_resident(amj_ctx_t* ctx, const amj_relation_t* R, amj_trie_t** out) {
   if (!ctx || !R || !out) return AMJ_ERR_ARG;
   *out = nullptr;
-  if (R->num_cols != 2) {
-    set_error(ctx, AMJ_ERR_ARG, "a trie is built from a two-column relation");
+  if (R->num_cols != 2 || R->host) {
+    set_error(ctx, AMJ_ERR_ARG, "a trie is built from a two-column device relation");
     return AMJ_ERR_ARG;
   }
   std::unique_ptr<amj_trie> t(new amj_trie());

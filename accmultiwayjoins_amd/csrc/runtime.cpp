@@ -78,6 +78,11 @@ int build_hash_table(amj_ctx_t* ctx, const amj_relation* rel, uint32_t key_col, 
     set_error(ctx, AMJ_ERR_ARG, "hash build: bad relation or key column");
     return AMJ_ERR_ARG;
   }
+  if (rel->host) {
+    set_error(ctx, AMJ_ERR_ARG, "hash build: the relation is host-resident (only a probe "
+                                "relation may stay on the host)");
+    return AMJ_ERR_ARG;
+  }
   if (rel->rows >= (1ull << 32)) {
     set_error(ctx, AMJ_ERR_ARG, "hash build: a build relation must have fewer than 2^32 rows");
     return AMJ_ERR_ARG;
@@ -162,6 +167,7 @@ int make_plan(amj_ctx_t* ctx, const ProbeUse& probe, const std::vector<TableUse>
     return -1;
   };
   plan->probe_n = probe.rel->rows;
+  plan->probe_on_host = probe.rel->host ? 1 : 0;
   plan->probe_ncols = (int)probe.rel->num_cols;
   for (uint32_t c = 0; c < probe.rel->num_cols; ++c) {
     plan->probe_cols[c] = probe.rel->cols[c];
@@ -361,6 +367,7 @@ int execute_plan(amj_ctx_t* ctx, Plan& plan, const std::vector<uint32_t>& attr_i
   std::memset(res, 0, sizeof(*res));
   res->num_cols = (uint32_t)plan.n_attrs;
   for (int a = 0; a < plan.n_attrs; ++a) res->attr_ids[a] = attr_ids[a];
+  if (o->ooc || plan.probe_on_host) return run_ooc(ctx, plan, o, res);
   if (o->fib && fib_eligible(plan) && window_plan_supported(plan)) {
     const int rc = run_fib(ctx, plan, o, res);
     if (rc != AMJ_ERR_STATE) return rc;

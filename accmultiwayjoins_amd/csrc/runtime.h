@@ -41,6 +41,7 @@ struct amj_relation {
   int32_t* cols[AMJ_MAX_PROBE_COLS] = {nullptr, nullptr};
   uint32_t attr_ids[AMJ_MAX_PROBE_COLS] = {0, 0};
   bool owned = false;
+  bool host = false;  // cols are host memory (out-of-core probe relation)
 };
 
 struct amj_hash_table {
@@ -130,5 +131,8 @@ int make_pattern_plan(amj_ctx_t* ctx, const amj_relation* edges, const amj_hash_
                       int query, Plan* plan, std::vector<uint32_t>* ids);
 // Persistent probe with work sharing (o->ws) and/or direct result output (o->dro).
 int run_dynamic(amj_ctx_t* ctx, Plan& P, const amj_opts_t* o, amj_result_t* res);
+// Out-of-core probe: the probe relation streamed from host memory in slices (o->ooc, or a
+// host-resident probe relation).
+int run_ooc(amj_ctx_t* ctx, Plan& P, const amj_opts_t* o, amj_result_t* res);
 
 }  // namespace amj

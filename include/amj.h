@@ -61,8 +61,13 @@ typedef struct {
   uint64_t rows;
   const int32_t* const* cols;   /* num_cols column pointers */
   const uint32_t* attr_ids;     /* num_cols attribute ids (may be NULL for TRI/FOUR edges) */
-  int on_device;                /* 1: cols are device pointers that stay valid; 0: host */
+  int on_device;                /* 0: host memory, copied to the device at creation;
+                                   1: device pointers that stay valid;
+                                   AMJ_REL_HOST: host memory that stays there (pinned memory
+                                   from amj_host_alloc streams fastest): a probe relation
+                                   used out of core, read in slices */
 } amj_rel_t;
+#define AMJ_REL_HOST 2
 
 /* Query options: the exec-AMHJ switches (exec_AMHJ.cu:216-245) plus this library's knobs. */
 typedef struct {
@@ -109,12 +114,19 @@ typedef struct {
 #define AMJ_PATH_LFTJ_BFS 6
 #define AMJ_PATH_WS_DRO 7
 #define AMJ_PATH_BATCH 8        /* many queries per launch (amj_batch_run) */
+#define AMJ_PATH_OOC 9          /* probe relation streamed in slices (opts.ooc / AMJ_REL_HOST) */
 
 /* ---- context ------------------------------------------------------------------------ */
 int amj_ctx_create(int device, amj_ctx_t** out);
 void amj_ctx_destroy(amj_ctx_t* ctx);
 const char* amj_last_error(const amj_ctx_t* ctx);
 int amj_sync(amj_ctx_t* ctx);
+/* Pinned host memory (hipHostMalloc) for relations kept on the host (AMJ_REL_HOST): the
+ * out-of-core probe streams it to the device at PCIe rate, overlapped with the probe. */
+int amj_host_alloc(size_t bytes, void** out);
+/* Device bytes held by the context now and at its peak (the HBM budget counts the former). */
+int amj_memory_info(amj_ctx_t* ctx, uint64_t* allocated, uint64_t* peak);
+void amj_host_free(void* p);
 int amj_device_info(amj_ctx_t* ctx, char* name, size_t name_len, uint64_t* hbm_bytes,
                     int* cu_count);
 void amj_default_opts(amj_opts_t* opts);

@@ -35,7 +35,8 @@ def golden(name):
     return next(r for r in json.load(open(GOLDEN))["datasets"] if r["name"] == name)
 
 
-@pytest.mark.parametrize("flags", [["0", "0", "0", "0"], ["0", "1", "1", "1"]])
+@pytest.mark.parametrize("flags", [["0", "0", "0", "0"], ["0", "1", "1", "1"],
+                                   ["1", "0", "0", "0"], ["1", "1", "1", "0"]])
 def test_exec_amhj_normal_matches_reference_count(tmp_path, flags):
     rec = golden("chain3_4096_keys1024")
     rels = G.chain_relations(rec["sizes"], rec["key_range"], seed=rec["seed"])
