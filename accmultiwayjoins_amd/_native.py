@@ -20,7 +20,7 @@ STATUS = {
     -4: "AMJ_ERR_NODEV", -5: "AMJ_ERR_STATE",
 }
 PATHS = {1: "plain", 2: "fib", 3: "ws", 4: "dro", 5: "lftj-dfs", 6: "lftj-bfs", 7: "ws-dro", 8: "batch",
-         9: "ooc"}
+         9: "ooc", 10: "lftj-ooc"}
 
 QUERY = {"NORMAL": 0, "TRI": 1, "FOUR": 2}
 AMJ_REL_HOST = 2
@@ -155,6 +155,7 @@ SIGNATURES = [
      [C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p, C.c_void_p,
       C.POINTER(C.c_void_p)]),
     ("alftj_trie_save", C.c_int, [C.c_void_p, C.c_void_p, C.c_char_p]),
+    ("alftj_trie_offload", C.c_int, [C.c_void_p, C.c_void_p]),
     ("alftj_trie_load", C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_void_p)]),
     ("amj_db_parse_name", C.c_int,
      [C.c_char_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),

@@ -170,6 +170,10 @@ class Trie:
         self.level0, self.children = n0.value, n1.value
         self.max_children, self.build_ms = mx.value, ms.value
 
+    def offload(self):
+        """Move the trie to pinned host memory; queries over it then run out of core."""
+        self.ctx._check(N.lib().alftj_trie_offload(self.ctx.h, self.h))
+
     def save(self, path: str):
         """Write the trie as a .tr file (IndexedTrie::serialization layout)."""
         self.ctx._check(N.lib().alftj_trie_save(self.ctx.h, self.h, path.encode()))

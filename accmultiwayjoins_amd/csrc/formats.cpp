@@ -137,9 +137,9 @@ int alftj_trie_save(amj_ctx_t* ctx, const amj_trie_t* t, const char* path) {
   std::vector<int32_t> d0(n0), d1(n1);
   std::vector<uint32_t> off32(n0 + 1);
   hipStream_t s = ctx->stream;
-  if (n0) AMJ_HIP_TRY(ctx, hipMemcpyAsync(d0.data(), t->dev.d0, n0 * 4, hipMemcpyDeviceToHost, s));
-  if (n1) AMJ_HIP_TRY(ctx, hipMemcpyAsync(d1.data(), t->dev.d1, n1 * 4, hipMemcpyDeviceToHost, s));
-  AMJ_HIP_TRY(ctx, hipMemcpyAsync(off32.data(), t->dev.off, (n0 + 1) * 4, hipMemcpyDeviceToHost, s));
+  if (n0) AMJ_HIP_TRY(ctx, hipMemcpyAsync(d0.data(), t->dev.view.d0, n0 * 4, hipMemcpyDefault, s));
+  if (n1) AMJ_HIP_TRY(ctx, hipMemcpyAsync(d1.data(), t->dev.view.d1, n1 * 4, hipMemcpyDefault, s));
+  AMJ_HIP_TRY(ctx, hipMemcpyAsync(off32.data(), t->dev.view.off, (n0 + 1) * 4, hipMemcpyDefault, s));
   AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
   std::vector<uint64_t> off(off32.begin(), off32.end());
   File f;

@@ -59,6 +59,23 @@ struct LOut {
 };
 hipError_t launch_lftj_write(const LPlan& P, const uint64_t* unit_base, const LOut& cols,
                              bool hash, uint64_t* block_sums, uint64_t* n_blocks, hipStream_t s);
+// Work-sharing variants: `grid` persistent workgroups claim units from *next (zeroed).
+hipError_t launch_lftj_count_ws(const LPlan& P, uint64_t* counts, unsigned long long* next,
+                                unsigned grid, hipStream_t s);
+hipError_t launch_lftj_write_ws(const LPlan& P, const uint64_t* base, const LOut& cols, bool hash,
+                                uint64_t* block_sums, unsigned long long* next, unsigned grid,
+                                hipStream_t s);
+
+// Level-0 arrays of a relation readable from the host.
+struct HostView {
+  const int32_t* d0;
+  const uint32_t* off;
+  uint32_t n0;
+};
+// Out-of-core evaluation (lftj_ooc.hip): per combination of level-0 value ranges, count and
+// checksum summed; parts_hint ranges per attribute (0: from the budget).
+int run_lftj_ooc(amj_ctx_t* ctx, const LPlan& P, const HostView* hv, const amj_opts_t* o,
+                 uint64_t parts_hint, amj_result_t* res, uint64_t* combos_out);
 
 // BFS engine (lftj_bfs.hip): count and checksum, level by level within mem_budget bytes.
 int run_lftj_bfs(amj_ctx_t* ctx, const LPlan& P, uint64_t mem_budget, uint64_t* count,
@@ -69,6 +86,10 @@ int run_lftj_bfs(amj_ctx_t* ctx, const LPlan& P, uint64_t mem_budget, uint64_t* 
 // A built trie as handed out by the C ABI: the device trie plus the attribute ids of its
 // two levels (IndexedTrie::attr_list, LFTJ-GPU/IndexedTrie.cuh:27).
 struct amj_trie {
-  amj::TrieDev dev;
-  uint32_t attr_ids[2] = {0, 1};
+  amj::TrieDev dev;               // device arrays; after alftj_trie_offload the view points to
+  uint32_t attr_ids[2] = {0, 1};  // the pinned host copies below and the device ones are freed
+  bool host = false;
+  int32_t* h_d0 = nullptr;
+  uint32_t* h_off = nullptr;
+  int32_t* h_d1 = nullptr;
 };

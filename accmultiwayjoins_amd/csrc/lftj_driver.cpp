@@ -97,9 +97,25 @@ static int run_lftj_dfs(amj_ctx_t* ctx, const LPlan& P, const amj_opts_t* o, amj
   void *counts = nullptr, *tmp = nullptr, *bsum = nullptr;
   AMJ_TRY(scratch(ctx, SLOT_UNIT_COUNTS, (units + 1) * 8, &counts));
   AMJ_TRY(scratch(ctx, SLOT_SCAN_TEMP, scan_temp_elems(units + 1) * 8, &tmp));
+  // ws: a device-sized grid claiming units from counters (two: count pass, write pass)
+  const bool ws = o->ws != 0;
+  void* next = nullptr;
+  unsigned ws_grid = 0;
+  if (ws) {
+    AMJ_TRY(scratch(ctx, SLOT_FLAG, 256, &next));
+    ws_grid = (unsigned)std::max<uint64_t>(
+        1, std::min<uint64_t>((units + kWavesPerBlock - 1) / kWavesPerBlock,
+                              (uint64_t)ctx->props.multiProcessorCount * 8));
+  }
   AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[2], s));
   AMJ_HIP_TRY(ctx, hipMemsetAsync(counts, 0, (units + 1) * 8, s));
-  AMJ_HIP_TRY(ctx, launch_lftj_count(P, (uint64_t*)counts, s));
+  if (ws) {
+    AMJ_HIP_TRY(ctx, hipMemsetAsync(next, 0, 16, s));
+    AMJ_HIP_TRY(ctx, launch_lftj_count_ws(P, (uint64_t*)counts, (unsigned long long*)next,
+                                          ws_grid, s));
+  } else {
+    AMJ_HIP_TRY(ctx, launch_lftj_count(P, (uint64_t*)counts, s));
+  }
   AMJ_HIP_TRY(ctx, exclusive_scan_u64((uint64_t*)counts, (uint64_t*)counts, units + 1,
                                       (uint64_t*)tmp, s));
   uint64_t total = 0;
@@ -119,12 +135,18 @@ static int run_lftj_dfs(amj_ctx_t* ctx, const LPlan& P, const amj_opts_t* o, amj
     for (int a = 0; a < P.n_attrs; ++a) out.c[a] = res->cols[a];
   }
   const uint64_t nb = units == 0 ? 1 : (units + kWavesPerBlock - 1) / kWavesPerBlock;
-  AMJ_TRY(scratch(ctx, SLOT_BLOCK_SUMS, nb * 8, &bsum));
+  AMJ_TRY(scratch(ctx, SLOT_BLOCK_SUMS, std::max<uint64_t>(nb, ws_grid) * 8, &bsum));
   const bool hash = o->checksum || !out.c[0];
   uint64_t nb_used = 0;
   AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
-  AMJ_HIP_TRY(ctx, launch_lftj_write(P, (const uint64_t*)counts, out, hash, (uint64_t*)bsum,
-                                     &nb_used, s));
+  if (ws && units > 0) {
+    AMJ_HIP_TRY(ctx, launch_lftj_write_ws(P, (const uint64_t*)counts, out, hash, (uint64_t*)bsum,
+                                          (unsigned long long*)next + 1, ws_grid, s));
+    nb_used = ws_grid;
+  } else {
+    AMJ_HIP_TRY(ctx, launch_lftj_write(P, (const uint64_t*)counts, out, hash, (uint64_t*)bsum,
+                                       &nb_used, s));
+  }
   AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
   if (hash) AMJ_TRY(finish_checksum(ctx, (const uint64_t*)bsum, nb_used, &res->checksum));
   AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[5], s));
@@ -147,11 +169,42 @@ int lftj_execute(amj_ctx_t* ctx, const std::vector<LUse>& uses, const uint32_t* 
   AMJ_TRY(make_lplan(ctx, uses, order, na, &P));
   res->num_cols = na;
   for (uint32_t a = 0; a < na; ++a) res->attr_ids[a] = order[a];
-  if (algo == ALFTJ_DFS) return run_lftj_dfs(ctx, P, o, res);
-  if (algo != ALFTJ_BFS) {
+  if (algo != ALFTJ_DFS && algo != ALFTJ_BFS) {
     set_error(ctx, AMJ_ERR_ARG, "algo must be ALFTJ_BFS (0) or ALFTJ_DFS (1)");
     return AMJ_ERR_ARG;
   }
+  bool any_host = false;
+  for (const auto& u : uses) any_host = any_host || u.t->host;
+  if (o->ooc || any_host) {
+    // out of core: level-0 arrays readable from the host for the range cuts
+    std::vector<HostView> hv(uses.size());
+    std::vector<std::vector<int32_t>> d0s(uses.size());
+    std::vector<std::vector<uint32_t>> offs(uses.size());
+    for (size_t r = 0; r < uses.size(); ++r) {
+      const amj_trie* t = uses[r].t;
+      if (t->host) {
+        hv[r] = HostView{t->h_d0, t->h_off, t->dev.n0};
+        continue;
+      }
+      d0s[r].resize(t->dev.n0);
+      offs[r].resize((size_t)t->dev.n0 + 1);
+      if (t->dev.n0)
+        AMJ_HIP_TRY(ctx, hipMemcpyAsync(d0s[r].data(), t->dev.d0, (size_t)t->dev.n0 * 4,
+                                        hipMemcpyDeviceToHost, ctx->stream));
+      AMJ_HIP_TRY(ctx, hipMemcpyAsync(offs[r].data(), t->dev.off, ((size_t)t->dev.n0 + 1) * 4,
+                                      hipMemcpyDeviceToHost, ctx->stream));
+      hv[r] = HostView{d0s[r].data(), offs[r].data(), t->dev.n0};
+    }
+    AMJ_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+    uint64_t parts = 0;
+    if (o->ooc_slice_rows) {  // level-0 values per range
+      uint64_t mx = 0;
+      for (const auto& h : hv) mx = std::max<uint64_t>(mx, h.n0);
+      parts = std::max<uint64_t>(1, (mx + o->ooc_slice_rows - 1) / o->ooc_slice_rows);
+    }
+    return run_lftj_ooc(ctx, P, hv.data(), o, parts, res, nullptr);
+  }
+  if (algo == ALFTJ_DFS) return run_lftj_dfs(ctx, P, o, res);
   // BFS: count and checksum (LFTJ_BFS::evaluate reports a count only, LFTJ_BFS.cuh:457-500)
   AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
   AMJ_TRY(run_lftj_bfs(ctx, P, budget_left(ctx, o), &res->count, &res->checksum));
@@ -203,8 +256,51 @@ int alftj_trie_build(amj_ctx_t* ctx, const amj_rel_t* rel, amj_trie_t** out) {
 
 void alftj_trie_destroy(amj_ctx_t* ctx, amj_trie_t* t) {
   if (!t) return;
-  free_trie(ctx, &t->dev);
+  if (t->host) {
+    (void)hipHostFree(t->h_d0);
+    (void)hipHostFree(t->h_off);
+    (void)hipHostFree(t->h_d1);
+    t->dev = TrieDev{};
+  } else {
+    free_trie(ctx, &t->dev);
+  }
   delete t;
+}
+
+int alftj_trie_offload(amj_ctx_t* ctx, amj_trie_t* t) {
+  if (!ctx || !t) return AMJ_ERR_ARG;
+  if (t->host) return AMJ_OK;
+  const uint64_t n0 = t->dev.n0, n1 = t->dev.n1;
+  int32_t *d0 = nullptr, *d1 = nullptr;
+  uint32_t* off = nullptr;
+  if (hipHostMalloc((void**)&d0, std::max<uint64_t>(n0, 1) * 4, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&off, (n0 + 1) * 4, hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&d1, std::max<uint64_t>(n1, 1) * 4, hipHostMallocDefault) != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipHostFree(d0);
+    (void)hipHostFree(off);
+    (void)hipHostFree(d1);
+    set_error(ctx, AMJ_ERR_NOMEM, "pinned host allocation for the trie failed");
+    return AMJ_ERR_NOMEM;
+  }
+  hipStream_t s = ctx->stream;
+  if (n0) AMJ_HIP_TRY(ctx, hipMemcpyAsync(d0, t->dev.d0, n0 * 4, hipMemcpyDeviceToHost, s));
+  AMJ_HIP_TRY(ctx, hipMemcpyAsync(off, t->dev.off, (n0 + 1) * 4, hipMemcpyDeviceToHost, s));
+  if (n1) AMJ_HIP_TRY(ctx, hipMemcpyAsync(d1, t->dev.d1, n1 * 4, hipMemcpyDeviceToHost, s));
+  AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
+  const uint32_t max_run = t->dev.max_run;
+  const double ms = t->dev.build_ms;
+  free_trie(ctx, &t->dev);
+  t->host = true;
+  t->h_d0 = d0;
+  t->h_off = off;
+  t->h_d1 = d1;
+  t->dev.n0 = (uint32_t)n0;
+  t->dev.n1 = (uint32_t)n1;
+  t->dev.max_run = max_run;
+  t->dev.build_ms = ms;
+  t->dev.view = LRel{d0, off, d1, nullptr, (uint32_t)n0, 0};
+  return AMJ_OK;
 }
 
 int alftj_trie_info(amj_ctx_t* ctx, const amj_trie_t* t, uint64_t* level0, uint64_t* children,
@@ -223,14 +319,14 @@ int alftj_trie_copy_level(amj_ctx_t* ctx, const amj_trie_t* t, int level, int32_
   hipStream_t s = ctx->stream;
   if (level == 0) {
     if (values && t->dev.n0)
-      AMJ_HIP_TRY(ctx, hipMemcpyAsync(values, t->dev.d0, (uint64_t)t->dev.n0 * 4,
-                                      hipMemcpyDeviceToHost, s));
+      AMJ_HIP_TRY(ctx, hipMemcpyAsync(values, t->dev.view.d0, (uint64_t)t->dev.n0 * 4,
+                                      hipMemcpyDefault, s));
     if (offsets)
-      AMJ_HIP_TRY(ctx, hipMemcpyAsync(offsets, t->dev.off, ((uint64_t)t->dev.n0 + 1) * 4,
-                                      hipMemcpyDeviceToHost, s));
+      AMJ_HIP_TRY(ctx, hipMemcpyAsync(offsets, t->dev.view.off, ((uint64_t)t->dev.n0 + 1) * 4,
+                                      hipMemcpyDefault, s));
   } else if (values && t->dev.n1) {
-    AMJ_HIP_TRY(ctx, hipMemcpyAsync(values, t->dev.d1, (uint64_t)t->dev.n1 * 4,
-                                    hipMemcpyDeviceToHost, s));
+    AMJ_HIP_TRY(ctx, hipMemcpyAsync(values, t->dev.view.d1, (uint64_t)t->dev.n1 * 4,
+                                    hipMemcpyDefault, s));
   }
   AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
   return AMJ_OK;

@@ -115,6 +115,7 @@ typedef struct {
 #define AMJ_PATH_WS_DRO 7
 #define AMJ_PATH_BATCH 8        /* many queries per launch (amj_batch_run) */
 #define AMJ_PATH_OOC 9          /* probe relation streamed in slices (opts.ooc / AMJ_REL_HOST) */
+#define AMJ_PATH_LFTJ_OOC 10    /* triejoin per combination of level-0 value ranges */
 
 /* ---- context ------------------------------------------------------------------------ */
 int amj_ctx_create(int device, amj_ctx_t** out);
@@ -261,6 +262,12 @@ int alftj_trie_from_csr(amj_ctx_t* ctx, uint64_t n0, const int32_t* d0, const ui
 /* .tr files (IndexedTrie::serialization / init_with_deserialization, IndexedTrie.cuh:122-197):
  * u32 num_attrs, u64 data_len[], u32 attr_list[], int32 levels, u64 offsets. Two levels only. */
 int alftj_trie_save(amj_ctx_t* ctx, const amj_trie_t* trie, const char* path);
+/* Move a trie to pinned host memory (the device copy is freed). Queries over such tries run
+ * out of core (OOCWrapper, LFTJ-GPU/ooc_wrapper.cuh:14-427): per combination of level-0 value
+ * ranges, the slices streamed on a side stream; opts.ooc = 1 does the same for device tries,
+ * and opts.ooc_slice_rows sets the level-0 values per range (0: from the HBM budget). Out of
+ * core, alftj_run returns count and checksum (no materialised tuples). */
+int alftj_trie_offload(amj_ctx_t* ctx, amj_trie_t* trie);
 int alftj_trie_load(amj_ctx_t* ctx, const char* path, amj_trie_t** out);
 
 /* ---- files and result fingerprint (host side, no context needed) ----------------------- */

@@ -157,3 +157,49 @@ def test_attr_order_errors(ctx):
         ctx.lftj([t01], [0, 1, 2])
     # the context stays usable after errors
     assert ctx.lftj([t01, t01], [0, 1]).count == 2
+
+
+@pytest.mark.parametrize("query", ["TRI", "FOUR"])
+def test_work_sharing_dfs_same_arrays(ctx, query):
+    s, d = G.rmat_oriented(11, 16, seed=3)
+    a = ctx.lftj_pattern(s, d, query, "DFS", fetch=True)
+    b = ctx.lftj_pattern(s, d, query, "DFS", fetch=True, ws=1)
+    assert (a.count, a.checksum) == (b.count, b.checksum)
+    # units keep their scanned output rows whichever wave takes them
+    for x, y in zip(a.cols, b.cols):
+        assert np.array_equal(x, y)
+
+
+@pytest.mark.parametrize("query", ["TRI", "FOUR"])
+@pytest.mark.parametrize("algo", ALGOS)
+@pytest.mark.parametrize("slice_rows", [37, 300, 5000])
+def test_out_of_core_range_combinations(ctx, query, algo, slice_rows):
+    s, d = G.rmat_oriented(10, 16, seed=5)
+    want = O.triangles(s, d) if query == "TRI" else O.four_cliques(s, d)
+    pattern = {"TRI": [(0, 1), (1, 2), (0, 2)],
+               "FOUR": [(0, 1), (1, 2), (2, 3), (0, 3), (0, 2), (1, 3)]}[query]
+    na = 3 if query == "TRI" else 4
+    t = ctx.trie([s, d])
+    # device trie sliced (opts.ooc), then the same trie offloaded to host memory
+    r = ctx.lftj([t] * len(pattern), list(range(na)), algo, trie_attrs=pattern, ooc=1,
+                 ooc_slice_rows=slice_rows, materialize=False)
+    assert (r.count, r.checksum) == want and r.path == "lftj-ooc"
+    t.offload()
+    r2 = ctx.lftj([t] * len(pattern), list(range(na)), algo, trie_attrs=pattern,
+                  ooc_slice_rows=slice_rows, materialize=False)
+    assert (r2.count, r2.checksum) == want and r2.detail == r.detail
+    assert r.detail > 1 or slice_rows >= t.level0  # several range combinations ran
+    t3 = ctx.trie([s, d])
+    t3.offload()
+    r3 = ctx.lftj([t3] * len(pattern), list(range(na)), algo, trie_attrs=pattern,
+                  materialize=False)  # ranges from the budget: everything fits, one combination
+    assert (r3.count, r3.checksum) == want
+
+
+def test_out_of_core_multiway_kat(ctx):
+    from test_oracle import KAT_TRIANGLE, kat_relations
+    tries = [ctx.trie(cols, ids) for cols, ids in kat_relations()]
+    for t in tries:
+        t.offload()
+    r = ctx.lftj(tries, [0, 1, 2], "DFS", ooc_slice_rows=1, materialize=False)
+    assert (r.count, r.checksum) == (5, O.checksum(KAT_TRIANGLE))
