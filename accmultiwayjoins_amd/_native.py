@@ -107,6 +107,10 @@ SIGNATURES = [
     ("amj_result_free", C.c_int, [C.c_void_p, C.POINTER(AmjResult)]),
     ("amj_result_copy_col", C.c_int,
      [C.c_void_p, C.POINTER(AmjResult), C.c_uint32, C.c_uint64, C.c_uint64, C.c_void_p]),
+    ("amj_autotune", C.c_int,
+     [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(AmjOpts),
+      C.POINTER(C.c_double)]),
+    ("amj_autotune_forget", None, [C.c_void_p]),
     ("amj_batch_create", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
     ("amj_batch_destroy", None, [C.c_void_p]),
     ("amj_batch_add", C.c_int,

@@ -348,6 +348,19 @@ class Context:
                                           N.ALGO[algo], C.byref(o), C.byref(res)))
         return self._finish(res, fetch)
 
+    def autotune(self, probe: Relation, tables: Sequence[HashTable], **opts) -> dict:
+        """Engine options chosen by timing candidates on a sample of the probe relation
+        (amj_autotune); pass them back to execute()."""
+        hts = (C.c_void_p * len(tables))(*[t.h.value for t in tables])
+        o = _opts(**opts)
+        ms = C.c_double()
+        self._check(N.lib().amj_autotune(self.h, probe.h, hts, len(tables), C.byref(o),
+                                         C.byref(ms)))
+        return {"ws": o.ws, "dro": o.dro, "fib": o.fib, "out_chunk_rows": o.out_chunk_rows,
+                "fib_suffix": o.fib_suffix, "ws_split_threshold": o.ws_split_threshold,
+                "materialize": bool(o.materialize), "checksum": bool(o.checksum),
+                "sample_ms": ms.value}
+
     def execute_pattern(self, edges: Relation, table: HashTable, query: str = "TRI",
                         fetch: bool = False, **opts) -> JoinResult:
         """TRI / FOUR over a resident edge relation and its table (key 0, sort_vals=True)."""

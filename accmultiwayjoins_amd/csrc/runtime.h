@@ -33,6 +33,12 @@ struct amj_ctx {
     uint32_t refs = 0;
   };
   std::map<void*, Slab> slabs;
+  // Autotuned options per query shape (table and probe handles, rows, materialise).
+  struct Tuned {
+    amj_opts_t o;
+    double ms;
+  };
+  std::map<std::vector<uint64_t>, Tuned> tuned;
 };
 
 struct amj_relation {

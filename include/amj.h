@@ -167,6 +167,18 @@ int amj_result_free(amj_ctx_t* ctx, amj_result_t* res);
 int amj_result_copy_col(amj_ctx_t* ctx, const amj_result_t* res, uint32_t col, uint64_t row0,
                         uint64_t n, int32_t* host_out);
 
+/* ---- autotuner ---------------------------------------------------------------------------- */
+/* No counterpart in the reference (its schedule is fixed at compile time, MHJ-GPU/conf.h).
+ * Times candidate engine options on an evenly strided sample (at most 4 M rows) of the probe
+ * relation with the caller's materialise setting: the unit engine, the persistent engine with
+ * small and large tasks, direct output, and for frequent-item chains the output-window size
+ * (16 Ki / 64 Ki / 256 Ki rows) with and without suffix tables. The fastest is written into
+ * *opts (materialize and checksum kept) and remembered in the context for the same query
+ * shape; tuned_ms is its device time on the sample. */
+int amj_autotune(amj_ctx_t* ctx, const amj_relation_t* probe, amj_hash_table_t* const* hts,
+                 uint32_t n_hts, amj_opts_t* opts, double* tuned_ms);
+void amj_autotune_forget(amj_ctx_t* ctx);
+
 /* ---- batches: many AMHJ queries per launch --------------------------------------------- */
 /* No counterpart in the reference (one query per process, exec_AMHJ.cu). Queries over
  * resident relations and hash tables are queued, then run together: one count launch and one

@@ -185,3 +185,18 @@ def test_dynamic_paths_on_skew_and_patterns(ctx, opts):
     ctx.free(r3)
     r4 = ctx.pattern_join(s, d, "FOUR", materialize=False, **opts)
     assert (r4.count, r4.checksum) == (n4, c4)
+
+
+def test_autotune_choice_gives_the_same_result(ctx):
+    rels = G.chain_relations([1 << 20, 1 << 18, 1 << 18, 1 << 18], 1 << 16, seed=42, zipf_s=1.0,
+                             payload_range=1 << 31)
+    R = [ctx.relation(c, ids) for c, ids in rels]
+    H = [ctx.hash_table(r) for r in R[1:]]
+    want = ctx.execute(R[0], H, materialize=False)
+    tuned = ctx.autotune(R[0], H, materialize=True)
+    opts = {k: v for k, v in tuned.items() if k != "sample_ms"}
+    r = ctx.execute(R[0], H, **opts)
+    assert (r.count, r.checksum) == (want.count, want.checksum)
+    ctx.free(r)
+    again = ctx.autotune(R[0], H, materialize=True)  # remembered for this query shape
+    assert again == tuned
