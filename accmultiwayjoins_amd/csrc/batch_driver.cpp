@@ -74,6 +74,10 @@ static int batch_add_plan(amj_batch_t* b, Plan& plan, std::vector<uint32_t>& ids
 int amj_batch_add(amj_batch_t* b, const amj_relation_t* probe, amj_hash_table_t* const* hts,
                   uint32_t n_hts, const amj_opts_t* opts, uint32_t* id) {
   if (!b || !probe || !hts) return AMJ_ERR_ARG;
+  if (probe->host) {
+    set_error(b->ctx, AMJ_ERR_ARG, "batched queries need a device-resident probe relation");
+    return AMJ_ERR_ARG;
+  }
   ProbeUse pu{};
   pu.rel = probe;
   for (uint32_t c = 0; c < probe->num_cols; ++c) pu.attr_ids[c] = probe->attr_ids[c];
