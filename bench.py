@@ -208,11 +208,23 @@ def main():
     ap.add_argument("--serve-max-rows", type=int, default=2_000_000)
     ap.add_argument("--serve-rate", type=float, default=4000.0)
     ap.add_argument("--tri-scale", type=int, default=22)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device to device three times (torch), to scale the "
+                         "FETCH_SIZE / WRITE_SIZE counters of a profiled run, then exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        import torch
+        src = torch.ones(1 << 28, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("[bench] copied 3 x 1 GiB")
+        return
 
     from accmultiwayjoins_amd import Context
     from accmultiwayjoins_amd import generators as G
