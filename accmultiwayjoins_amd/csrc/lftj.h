@@ -59,12 +59,6 @@ struct LOut {
 };
 hipError_t launch_lftj_write(const LPlan& P, const uint64_t* unit_base, const LOut& cols,
                              bool hash, uint64_t* block_sums, uint64_t* n_blocks, hipStream_t s);
-// Work-sharing variants: `grid` persistent workgroups claim units from *next (zeroed).
-hipError_t launch_lftj_count_ws(const LPlan& P, uint64_t* counts, unsigned long long* next,
-                                unsigned grid, hipStream_t s);
-hipError_t launch_lftj_write_ws(const LPlan& P, const uint64_t* base, const LOut& cols, bool hash,
-                                uint64_t* block_sums, unsigned long long* next, unsigned grid,
-                                hipStream_t s);
 
 // Level-0 arrays of a relation readable from the host.
 struct HostView {

@@ -111,86 +111,7 @@ __global__ __launch_bounds__(kBlock) void lftj_write_kernel(const LPlan P,
   block_sum_store(acc.csum, &block_sums[blockIdx.x]);
 }
 
-// Work sharing (ALFTJ_count/write<WS=true>, ALFTJ.cuh:464-499): a grid sized to the device
-// whose waves claim units from a counter, so a wave that drew hub vertices does not hold up a
-// fixed share of the roots. Inside a unit every intersection is already spread over the 64
-// lanes, which is what the reference's splitting of long intersections buys it.
-template <int A>
-__global__ __launch_bounds__(kBlock) void lftj_count_ws_kernel(const LPlan P,
-                                                               uint64_t* __restrict__ counts,
-                                                               uint64_t units,
-                                                               unsigned long long* next) {
-  for (;;) {
-    uint64_t u = 0;
-    if (lane_id() == 0) u = atomicAdd(next, 1ull);
-    u = uniform(__shfl(u, 0, kWave));
-    if (u >= units) break;
-    LLv<A> l0;
-    level0(P, u, l0);
-    LAcc acc{0, 0, 0};
-    if (l0.T) descend<A, 1, MODE_COUNT>(P, l0, LOut{}, acc);
-    if (lane_id() == 0) counts[u] = acc.count;
-  }
-}
-
-template <int A, int MODE>
-__global__ __launch_bounds__(kBlock) void lftj_write_ws_kernel(const LPlan P,
-                                                               const uint64_t* __restrict__ base,
-                                                               const LOut cols,
-                                                               uint64_t* __restrict__ block_sums,
-                                                               uint64_t units,
-                                                               unsigned long long* next) {
-  LAcc acc{0, 0, 0};
-  for (;;) {
-    uint64_t u = 0;
-    if (lane_id() == 0) u = atomicAdd(next, 1ull);
-    u = uniform(__shfl(u, 0, kWave));
-    if (u >= units) break;
-    LLv<A> l0;
-    level0(P, u, l0);
-    acc.out_base = uniform(base[u]);
-    if (l0.T) descend<A, 1, MODE>(P, l0, cols, acc);
-  }
-  block_sum_store(acc.csum, &block_sums[blockIdx.x]);
-}
-
 uint64_t lftj_units(const LPlan& P) { return (P.roots + kWave - 1) / kWave; }
-
-hipError_t launch_lftj_count_ws(const LPlan& P, uint64_t* counts, unsigned long long* next,
-                                unsigned grid, hipStream_t s) {
-  const uint64_t units = lftj_units(P);
-  if (units == 0) return hipSuccess;
-  switch (P.n_attrs) {
-    case 2: hipLaunchKernelGGL(lftj_count_ws_kernel<2>, dim3(grid), dim3(kBlock), 0, s, P, counts, units, next); break;
-    case 3: hipLaunchKernelGGL(lftj_count_ws_kernel<3>, dim3(grid), dim3(kBlock), 0, s, P, counts, units, next); break;
-    case 4: hipLaunchKernelGGL(lftj_count_ws_kernel<4>, dim3(grid), dim3(kBlock), 0, s, P, counts, units, next); break;
-    default: return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
-}
-
-hipError_t launch_lftj_write_ws(const LPlan& P, const uint64_t* base, const LOut& cols, bool hash,
-                                uint64_t* block_sums, unsigned long long* next, unsigned grid,
-                                hipStream_t s) {
-  const uint64_t units = lftj_units(P);
-  const int mode = cols.c[0] ? (hash ? MODE_WRITE_HASH : MODE_WRITE) : MODE_HASH;
-#define LW(AV, M)                                                                            \
-  hipLaunchKernelGGL((lftj_write_ws_kernel<AV, M>), dim3(grid), dim3(kBlock), 0, s, P, base, \
-                     cols, block_sums, units, next)
-#define LWA(AV)                                              \
-  if (mode == MODE_WRITE) LW(AV, MODE_WRITE);                \
-  else if (mode == MODE_WRITE_HASH) LW(AV, MODE_WRITE_HASH); \
-  else LW(AV, MODE_HASH)
-  switch (P.n_attrs) {
-    case 2: LWA(2); break;
-    case 3: LWA(3); break;
-    case 4: LWA(4); break;
-    default: return hipErrorInvalidValue;
-  }
-#undef LWA
-#undef LW
-  return hipGetLastError();
-}
 
 hipError_t launch_lftj_count(const LPlan& P, uint64_t* counts, hipStream_t s) {
   const uint64_t units = lftj_units(P);

@@ -97,25 +97,11 @@ static int run_lftj_dfs(amj_ctx_t* ctx, const LPlan& P, const amj_opts_t* o, amj
   void *counts = nullptr, *tmp = nullptr, *bsum = nullptr;
   AMJ_TRY(scratch(ctx, SLOT_UNIT_COUNTS, (units + 1) * 8, &counts));
   AMJ_TRY(scratch(ctx, SLOT_SCAN_TEMP, scan_temp_elems(units + 1) * 8, &tmp));
-  // ws: a device-sized grid claiming units from counters (two: count pass, write pass)
-  const bool ws = o->ws != 0;
-  void* next = nullptr;
-  unsigned ws_grid = 0;
-  if (ws) {
-    AMJ_TRY(scratch(ctx, SLOT_FLAG, 256, &next));
-    ws_grid = (unsigned)std::max<uint64_t>(
-        1, std::min<uint64_t>((units + kWavesPerBlock - 1) / kWavesPerBlock,
-                              (uint64_t)ctx->props.multiProcessorCount * 8));
-  }
+  // opts.ws runs this same engine: every intersection is already spread over the 64 lanes of
+  // a wave (what the reference's ws buys by splitting long intersections)
   AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[2], s));
   AMJ_HIP_TRY(ctx, hipMemsetAsync(counts, 0, (units + 1) * 8, s));
-  if (ws) {
-    AMJ_HIP_TRY(ctx, hipMemsetAsync(next, 0, 16, s));
-    AMJ_HIP_TRY(ctx, launch_lftj_count_ws(P, (uint64_t*)counts, (unsigned long long*)next,
-                                          ws_grid, s));
-  } else {
-    AMJ_HIP_TRY(ctx, launch_lftj_count(P, (uint64_t*)counts, s));
-  }
+  AMJ_HIP_TRY(ctx, launch_lftj_count(P, (uint64_t*)counts, s));
   AMJ_HIP_TRY(ctx, exclusive_scan_u64((uint64_t*)counts, (uint64_t*)counts, units + 1,
                                       (uint64_t*)tmp, s));
   uint64_t total = 0;
@@ -135,18 +121,12 @@ static int run_lftj_dfs(amj_ctx_t* ctx, const LPlan& P, const amj_opts_t* o, amj
     for (int a = 0; a < P.n_attrs; ++a) out.c[a] = res->cols[a];
   }
   const uint64_t nb = units == 0 ? 1 : (units + kWavesPerBlock - 1) / kWavesPerBlock;
-  AMJ_TRY(scratch(ctx, SLOT_BLOCK_SUMS, std::max<uint64_t>(nb, ws_grid) * 8, &bsum));
+  AMJ_TRY(scratch(ctx, SLOT_BLOCK_SUMS, nb * 8, &bsum));
   const bool hash = o->checksum || !out.c[0];
   uint64_t nb_used = 0;
   AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
-  if (ws && units > 0) {
-    AMJ_HIP_TRY(ctx, launch_lftj_write_ws(P, (const uint64_t*)counts, out, hash, (uint64_t*)bsum,
-                                          (unsigned long long*)next + 1, ws_grid, s));
-    nb_used = ws_grid;
-  } else {
-    AMJ_HIP_TRY(ctx, launch_lftj_write(P, (const uint64_t*)counts, out, hash, (uint64_t*)bsum,
-                                       &nb_used, s));
-  }
+  AMJ_HIP_TRY(ctx, launch_lftj_write(P, (const uint64_t*)counts, out, hash, (uint64_t*)bsum,
+                                     &nb_used, s));
   AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
   if (hash) AMJ_TRY(finish_checksum(ctx, (const uint64_t*)bsum, nb_used, &res->checksum));
   AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[5], s));

@@ -159,6 +159,9 @@ def test_attr_order_errors(ctx):
     assert ctx.lftj([t01, t01], [0, 1]).count == 2
 
 
+@pytest.mark.skip(reason="the persistent unit-claiming ALFTJ ws kernels faulted on their first "
+                         "GPU run (exec-LFTJ DFS ws=1, R-MAT scale 11); cause not found, the "
+                         "kernels were taken out and ws=1 runs the unit engine")
 @pytest.mark.parametrize("query", ["TRI", "FOUR"])
 def test_work_sharing_dfs_same_arrays(ctx, query):
     s, d = G.rmat_oriented(11, 16, seed=3)
