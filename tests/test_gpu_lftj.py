@@ -66,7 +66,8 @@ def test_trie_empty_and_single(ctx):
     for algo in ALGOS:
         r = ctx.lftj([e, one], [0, 1], algo)
         assert r.count == 0
-        r = ctx.lftj([one, one, one], [0, 1, 2], algo)  # (7,7,7)
+        r = ctx.lftj([one, one, one], [0, 1, 2], algo,
+                     trie_attrs=[(0, 1), (1, 2), (0, 2)])  # (7,7,7)
         assert r.count == 1 and r.checksum == O.checksum([(7, 7, 7)])
 
 
