@@ -158,6 +158,29 @@ def brute_join(relations):
     return sorted(tuple(t[a] for a in range(A)) for t in rows)
 
 
+def brute_natural_join(relations):
+    """Natural join of relations [(cols, attr_ids)] by nested loops in any attribute layout
+    (products included); returns the sorted list of result tuples in attribute order 0..A-1."""
+    rows = [dict()]
+    for cols, ids in relations:
+        tuples = [tuple(int(c[r]) for c in cols) for r in range(len(cols[0]))]
+        nxt = []
+        for t in rows:
+            for vals in tuples:
+                u = dict(t)
+                ok = True
+                for a, v in zip(ids, vals):
+                    if a in u and u[a] != v:
+                        ok = False
+                        break
+                    u[a] = v
+                if ok:
+                    nxt.append(u)
+        rows = nxt
+    A = len({a for _, ids in relations for a in ids})
+    return sorted(tuple(t[a] for a in range(A)) for t in rows)
+
+
 def sort_tuples(arr: np.ndarray) -> np.ndarray:
     if arr.size == 0:
         return arr

@@ -91,7 +91,7 @@ def test_small_queries_equal_brute_force(ctx, name, algo):
         n = int(rng.integers(50, 300))
         cols = _dedupe([rng.integers(0, 12, n), rng.integers(0, 12, n)])
         rels.append((cols, [x, y]))
-    want = O.brute_join(rels)
+    want = O.brute_natural_join(rels)
     A = 1 + max(max(q) for q in QUERIES[name])
     tries = [ctx.trie(c, ids) for c, ids in rels]
     r = ctx.lftj(tries, list(range(A)), algo, fetch=True)
