@@ -125,7 +125,7 @@ def serving_leg(G, n_queries, max_rows, rate, seed=42):
         k = 3 + i % 3
         sizes = [int(10 ** rng.uniform(4, math.log10(max_rows))) for _ in range(k)]
         sizes[1:] = [max(1000, x // 4) for x in sizes[1:]]
-        keys = max(1, min(sizes[1:]))
+        keys = max(sizes[1:])  # every join fans out at most 1 on average: |result| <~ |probe|
         chains.append(G.chain_relations(sizes, keys, seed=seed + i, zipf_s=rng.uniform(0, 1.2),
                                         payload_range=1 << 30))
     graphs = [G.rmat_oriented(sc, 16, seed=seed + sc) for sc in (12, 13, 14, 15, 16)]
@@ -142,16 +142,19 @@ def serving_leg(G, n_queries, max_rows, rate, seed=42):
         with Server(0, max_batch=256, plain=plain) as srv:
             cids = [[srv.add_relation(c, a) for c, a in ch] for ch in chains]
             gids = [srv.add_relation([sd[0], sd[1]], (0, 1)) for sd in graphs]
-            tickets, lat, outs = [], [], []
+            tickets, lat, outs, errors = [], [], [], []
             done = threading.Event()
 
             def waiter():
-                for i in range(n_queries):
-                    while len(tickets) <= i:
-                        time.sleep(0.0001)
-                    cnt, cs, ms, _ = srv.wait(tickets[i])
-                    lat.append(ms)
-                    outs.append((cnt, cs))
+                try:
+                    for i in range(n_queries):
+                        while len(tickets) <= i:
+                            time.sleep(0.0001)
+                        cnt, cs, ms, _ = srv.wait(tickets[i])
+                        lat.append(ms)
+                        outs.append((cnt, cs))
+                except Exception as e:  # noqa: BLE001 - reported below
+                    errors.append(repr(e))
                 done.set()
 
             th = threading.Thread(target=waiter)
@@ -167,10 +170,18 @@ def serving_leg(G, n_queries, max_rows, rate, seed=42):
                     tickets.append(srv.submit(cids[k], materialize=True))
                 else:
                     tickets.append(srv.submit_pattern(gids[k], kind, materialize=True))
-            done.wait()
+            deadline = time.perf_counter() + 300
+            while not done.wait(timeout=20):  # progress lines while the stream drains
+                log(f"[bench] serving: {len(outs)}/{n_queries} answered")
+                if time.perf_counter() > deadline:
+                    raise SystemExit("bench: the serving leg did not finish within 300 s")
             el = time.perf_counter() - t0
             th.join()
+            if errors:
+                raise SystemExit("bench: a served query failed: " + errors[0])
             st = srv.stats()
+            log(f"[bench] serving ({'plain' if plain else 'batched'}): {n_queries} queries in "
+                f"{el:.2f} s")
         lat.sort()
         return {"queries_per_s": n_queries / el, "p50_ms": lat[len(lat) // 2],
                 "p99_ms": lat[min(len(lat) - 1, int(len(lat) * 0.99))],
