@@ -157,7 +157,7 @@ def serving_leg(G, n_queries, max_rows, rate, seed=42):
                     errors.append(repr(e))
                 done.set()
 
-            th = threading.Thread(target=waiter)
+            th = threading.Thread(target=waiter, daemon=True)
             th.start()
             t0 = time.perf_counter()
             due = t0
