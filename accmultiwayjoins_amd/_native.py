@@ -51,7 +51,8 @@ class AmjOpts(C.Structure):
         ("dro_capacity_rows", C.c_uint64),
         ("ws_split_threshold", C.c_uint32),
         ("fib_suffix", C.c_int),
-        ("reserved", C.c_uint32 * 6),
+        ("probe_order", C.c_int),
+        ("reserved", C.c_uint32 * 5),
     ]
 
 

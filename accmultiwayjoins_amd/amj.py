@@ -53,7 +53,7 @@ class JoinResult:
 
 
 def _opts(ooc=0, ws=0, dro=0, fib=0, materialize=True, checksum=True, hbm_budget_bytes=0,
-          ooc_slice_rows=0, out_chunk_rows=0, dro_capacity_rows=0,
+          ooc_slice_rows=0, out_chunk_rows=0, dro_capacity_rows=0, probe_order=0,
           ws_split_threshold=0, fib_suffix=0) -> N.AmjOpts:
     o = N.AmjOpts()
     N.lib().amj_default_opts(C.byref(o))
@@ -66,6 +66,7 @@ def _opts(ooc=0, ws=0, dro=0, fib=0, materialize=True, checksum=True, hbm_budget
     o.dro_capacity_rows = int(dro_capacity_rows)
     o.ws_split_threshold = int(ws_split_threshold)
     o.fib_suffix = int(fib_suffix)
+    o.probe_order = int(probe_order)
     return o
 
 

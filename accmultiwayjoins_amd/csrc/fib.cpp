@@ -65,6 +65,31 @@ int run_fib(amj_ctx_t* ctx, Plan& P, const amj_opts_t* o, amj_result_t* res) {
   const int lastv = P.val_mode[L - 1] == VAL_BIND ? 1 : 0;
   AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[2], s));
 
+  // 0. probe rows grouped by join key: rows that share a key are then neighbours, so their
+  // lookups, run records and the suffix rows they copy are served from cache (one stable
+  // radix sort on the key bytes, the other column carried along)
+  const uint64_t np = P.probe_n;
+  if (np > 0 && (o->probe_order == 2 || (o->probe_order == 0 && np >= (1ull << 20)))) {
+    int kcol = 0;
+    for (int c = 0; c < P.probe_ncols; ++c)
+      if (P.probe_attr[c] == P.key_attr[0]) kcol = c;
+    const int other = P.probe_ncols == 2 ? 1 - kcol : -1;
+    void *comp = nullptr, *alt = nullptr, *tmp = nullptr, *cols = nullptr;
+    AMJ_TRY(scratch(ctx, SLOT_SORT_KEYS, np * 8, &comp));
+    AMJ_TRY(scratch(ctx, SLOT_SORT_ALT, np * 8, &alt));
+    AMJ_TRY(scratch(ctx, SLOT_SORT_TEMP, radix_sort_temp_bytes(np), &tmp));
+    AMJ_TRY(scratch(ctx, SLOT_PROBE_SORTED, np * 4 * P.probe_ncols, &cols));
+    int32_t* sk = (int32_t*)cols;
+    int32_t* so = other >= 0 ? sk + np : nullptr;
+    AMJ_HIP_TRY(ctx, launch_pack_composite(P.probe_cols[kcol],
+                                           other >= 0 ? P.probe_cols[other] : nullptr, np, 0,
+                                           (uint64_t*)comp, s));
+    AMJ_HIP_TRY(ctx, radix_sort_u64((uint64_t*)comp, (uint64_t*)alt, np, 0xf0u, tmp, s));
+    AMJ_HIP_TRY(ctx, launch_unpack_composite((const uint64_t*)comp, np, 0, sk, so, s));
+    P.probe_cols[kcol] = sk;
+    if (other >= 0) P.probe_cols[other] = so;
+  }
+
   // 1. weights, prefixes and records, bottom-up (weights of table t need PW of table t+1)
   RunRef* rec[kMaxLevels] = {nullptr};
   uint64_t* pw[kMaxLevels] = {nullptr};

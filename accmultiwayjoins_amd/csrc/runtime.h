@@ -88,6 +88,7 @@ enum ScratchSlot {
   SLOT_TRIE1,
   SLOT_TRIE2,
   SLOT_TRIE3,
+  SLOT_PROBE_SORTED,  // frequent-item path: probe columns grouped by join key
   SLOT_DYN_CTL,    // persistent probe: control block, task records, task flags
   SLOT_DYN_TASKS,
   SLOT_DYN_READY,

@@ -84,7 +84,10 @@ typedef struct {
   uint64_t dro_capacity_rows;   /* dro: initial output allocation (0: sampled estimate) */
   uint32_t ws_split_threshold;  /* ws: split runs longer than this (0: default) */
   int fib_suffix;               /* fib: 0 size-based suffix tables, 1 none, 2 deepest that fits */
-  uint32_t reserved[6];
+  int probe_order;              /* fib: 0 group probe rows by join key when the probe has at
+                                   least 2^20 rows, 1 keep the input order (the unit engine's
+                                   output order), 2 always group */
+  uint32_t reserved[5];
 } amj_opts_t;
 
 /* Result of one join. Columns are device memory owned by the library until

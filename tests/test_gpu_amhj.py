@@ -16,9 +16,11 @@ pytestmark = pytest.mark.gpu
 GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "mhj_cpu_counts.json")))
 
 PATHS = [dict(), dict(fib=1), dict(fib=1, fib_suffix=1), dict(fib=1, fib_suffix=2),
+         dict(fib=1, probe_order=2),
          dict(ws=1), dict(ws=1, ws_split_threshold=64), dict(dro=1),
          dict(dro=1, dro_capacity_rows=64), dict(ws=1, dro=1, ws_split_threshold=128)]
-PATH_IDS = ["plain", "fib", "fib-nosuffix", "fib-deepsuffix", "ws", "ws-split64", "dro",
+PATH_IDS = ["plain", "fib", "fib-nosuffix", "fib-deepsuffix", "fib-grouped", "ws", "ws-split64",
+            "dro",
             "dro-overflow", "ws-dro"]
 
 
@@ -74,11 +76,17 @@ def test_fib_and_plain_write_identical_arrays(ctx):
     rels = G.chain_relations([20000, 5000, 7000, 3000], 300, seed=3, zipf_s=1.0)
     a = ctx.chain_join(rels, fetch=True)
     for suffix in (0, 1, 2):
-        b = ctx.chain_join(rels, fetch=True, fib=1, out_chunk_rows=1000, fib_suffix=suffix)
+        b = ctx.chain_join(rels, fetch=True, fib=1, out_chunk_rows=1000, fib_suffix=suffix,
+                           probe_order=1)
         assert a.path == "plain" and b.path == "fib"
         assert a.count == b.count > 0
         for ca, cb in zip(a.cols, b.cols):
             assert np.array_equal(ca, cb)
+        # grouped by join key the rows come out in another order, the same multiset
+        c = ctx.chain_join(rels, fetch=True, fib=1, out_chunk_rows=1000, fib_suffix=suffix,
+                           probe_order=2)
+        assert (c.count, c.checksum) == (a.count, a.checksum)
+        assert np.array_equal(O.sort_tuples(c.tuples()), O.sort_tuples(a.tuples()))
 
 
 def test_heavy_skew_single_key(ctx):
