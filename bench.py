@@ -210,6 +210,8 @@ def main():
     ap.add_argument("--ws", type=int, default=1)
     ap.add_argument("--dro", type=int, default=1)
     ap.add_argument("--fib", type=int, default=1)
+    ap.add_argument("--probe-order", type=int, default=0,
+                    help="0: group probe rows by join key (default), 1: keep their order")
     ap.add_argument("--cpu-sample-rows", type=int, default=1 << 21)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
@@ -252,7 +254,8 @@ def main():
     for s in skews:
         R = [ctx.relation(cols, ids) for cols, ids in inputs[s]]
         resident[s] = R
-    opts = dict(ws=args.ws, dro=args.dro, fib=args.fib, materialize=True, checksum=False)
+    opts = dict(ws=args.ws, dro=args.dro, fib=args.fib, materialize=True, checksum=False,
+                probe_order=args.probe_order)
 
     def step(record):
         tot = 0
