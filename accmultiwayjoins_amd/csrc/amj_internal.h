@@ -109,6 +109,15 @@ struct Plan {
 // ---------------------------------------------------------------------------------------
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
 
+// The dispatcher hands workgroup b of a launch to XCD b % 8. xcd_block(b, n) is a bijection of
+// [0, n) that gives every XCD one contiguous range of logical blocks, so the waves resident on
+// one XCD work on neighbouring data and share its L2 and address-translation caches.
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_block(uint32_t b, uint32_t n) {
+  const uint32_t q = n / kXcds, r = n % kXcds, x = b % kXcds, i = b / kXcds;
+  return x * q + (x < r ? x : r) + i;
+}
+
 __device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
                                    __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));

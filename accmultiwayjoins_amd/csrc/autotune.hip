@@ -99,7 +99,7 @@ int amj_autotune(amj_ctx_t* ctx, const amj_relation_t* probe, amj_hash_table_t* 
     std::vector<uint32_t> ids;
     AMJ_TRY(make_plan(ctx, pu, uses, &P, &ids, nullptr));
     if (fib_eligible(P) && window_plan_supported(P))
-      for (uint64_t w : {16384ull, 65536ull, 262144ull})
+      for (uint64_t w : {1024ull, 4096ull, 16384ull})
         for (int sfx : {0, 1}) add(1, 1, 1, w, sfx, 0);
   }
   double best = 1e300;

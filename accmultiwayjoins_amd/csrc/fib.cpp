@@ -21,7 +21,7 @@
 
 namespace amj {
 
-constexpr uint64_t kDefaultWindow = 65536;  // results per wave in the output-window writer
+constexpr uint64_t kDefaultWindow = 4096;  // results per window of the output-window writer
 
 // One launch of the output-window writer over plan Q; returns AMJ_ERR_STATE on overflow.
 static int window_pass(amj_ctx_t* ctx, const Plan& Q, const uint64_t* off, uint64_t total,

@@ -47,6 +47,12 @@ typedef __attribute__((address_space(3))) int32_t lds_i32;
 // at multiples of 64 rows, so a block never straddles two waves.
 constexpr int kStageRows = 2 * kWave;
 
+// Full 64-result batches the last-level emitter locates before it stores any of them.
+#ifndef AMJ_EMIT_GROUP
+#define AMJ_EMIT_GROUP 8
+#endif
+constexpr int kEmitGroup = AMJ_EMIT_GROUP;
+
 struct AccW {
   uint64_t csum;      // per-lane checksum partial
   uint64_t next_out;  // absolute row of the next result (uniform)
@@ -66,8 +72,28 @@ __device__ __forceinline__ void flush_block(const ChainOut& out, lds_i32* stage,
   const uint32_t slot = (uint32_t)row & (kStageRows - 1);
   if (lane < n) {
     int32_t* dst = out.base + row;
+#ifdef AMJ_WINDOW_NT_STORE
+    sfor<0, NA>([&](auto a) {
+      __builtin_nontemporal_store((int32_t)stage[a * kStageRows + slot],
+                                  &dst[(uint64_t)a * out.stride]);
+    });
+#else
     sfor<0, NA>([&](auto a) { dst[(uint64_t)a * out.stride] = stage[a * kStageRows + slot]; });
+#endif
   }
+  __asm__ __volatile__("" ::: "memory");
+}
+
+// Write the complete 64-row block b of the staging ring (no lane predicate, so the compiler
+// knows the wave issues exactly NA stores).
+template <int NA>
+__device__ __forceinline__ void flush_full_block(const ChainOut& out, lds_i32* stage,
+                                                 uint64_t b) {
+  __asm__ __volatile__("" ::: "memory");  // LDS ops of one wave complete in order
+  const uint64_t row = b * kWave + (uint32_t)lane_id();
+  const uint32_t slot = (uint32_t)row & (kStageRows - 1);
+  int32_t* dst = out.base + row;
+  sfor<0, NA>([&](auto a) { dst[(uint64_t)a * out.stride] = stage[a * kStageRows + slot]; });
   __asm__ __volatile__("" ::: "memory");
 }
 
@@ -159,8 +185,11 @@ struct Chain {
 
   // Results [c0, e1) of a last-level chunk; result g goes to absolute row e0 + g. Two batches
   // of 64 are located (parent search + suffix loads) before either is stored, so the second
-  // batch's loads are in flight while the first batch is shuffled and written. Stores are
-  // non-temporal: the result stream is never re-read by this kernel.
+  // batch's loads are in flight while the first batch is shuffled and written. While at least
+  // kEmitGroup full batches remain, they go as one group with no per-lane predicates: every
+  // lane loads, every batch completes exactly one 64-row block, so the compiler can wait for
+  // each batch's loads alone (a fixed count of memory operations issued after them) and the
+  // loads of the later batches of the group stay in flight while the first ones are stored.
   template <bool MAT, bool HASH>
   static __device__ __forceinline__ void emit(const Plan& P, const ChainLv<NB>& l, uint32_t c0,
                                               uint32_t e1, uint64_t e0, const ChainOut& out,
@@ -194,7 +223,32 @@ struct Chain {
         if ((rb >> 6) != (re >> 6)) flush_block<NA>(out, acc.stage, rb >> 6, (uint32_t)kWave);
       }
     };
-    for (uint32_t c = c0; c < e1; c += 2u * (uint32_t)kWave) {
+    uint32_t c = c0;
+    for (; e1 - c >= (uint32_t)(kEmitGroup * kWave); c += (uint32_t)(kEmitGroup * kWave)) {
+      int pg[kEmitGroup];
+      int32_t sg[kEmitGroup][NV > 0 ? NV : 1];
+      sfor<0, kEmitGroup>([&](auto i) {
+        pg[i] = (int)parent_of_slots(l.excl, l.len, c + (uint32_t)i * kWave);
+        if constexpr (NV > 0) {
+          const uint32_t row =
+              (uint32_t)__shfl(l.base, pg[i], kWave) + c + (uint32_t)i * kWave + lane_id();
+          sfor<0, NV>([&](auto k) { sg[i][k] = sfx[(uint64_t)k * sst + row]; });
+        }
+      });
+      sfor<0, kEmitGroup>([&](auto i) {
+        const uint32_t ci = c + (uint32_t)i * kWave;
+        int32_t v[NA];
+        sfor<0, NB>([&](auto a) { v[a] = __shfl(l.b[a], pg[i], kWave); });
+        if constexpr (NV > 0) sfor<0, NV>([&](auto k) { v[NB + k] = sg[i][k]; });
+        if (HASH) acc.csum += hash_chain<NA>(v);
+        if (MAT) {
+          const uint32_t slot = (uint32_t)(e0 + ci + lane_id()) & (kStageRows - 1);
+          sfor<0, NA>([&](auto a) { acc.stage[a * kStageRows + slot] = v[a]; });
+          flush_full_block<NA>(out, acc.stage, (e0 + ci) >> 6);
+        }
+      });
+    }
+    for (; c < e1; c += 2u * (uint32_t)kWave) {
       const uint32_t cb = c + (uint32_t)kWave;
       int pa, pb = 0;
       int32_t sa[NV > 0 ? NV : 1], sb[NV > 0 ? NV : 1];
@@ -242,13 +296,21 @@ __global__ __launch_bounds__(kBlock) void window_write_kernel(
   using C = Chain<L, PC, NV>;
   constexpr int NA = Chain<L, PC, NV>::NA;
   __shared__ int32_t stage_mem[MAT ? kWavesPerBlock * NA * kStageRows : 1];
-  const uint64_t w = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+#ifdef AMJ_WINDOW_LINEAR
+  const uint64_t blk = blockIdx.x;
+#else
+  const uint64_t blk = xcd_block(blockIdx.x, gridDim.x);  // neighbouring windows per XCD
+#endif
+  // windows w, w + n_slots, ...: the grid is capped (a launch holds fewer than 2^32 lanes)
+  const uint64_t n_slots = (uint64_t)gridDim.x * kWavesPerBlock;
   AccW acc{0, 0, 0, nullptr, false, false};
   acc.stage = (lds_i32*)&stage_mem[MAT ? (threadIdx.x / kWave) * NA * kStageRows : 0];
-  if (w < n_windows) {
+  for (uint64_t w = blk * kWavesPerBlock + threadIdx.x / kWave; w < n_windows && !acc.overflow;
+       w += n_slots) {
     const uint64_t g0 = w * window;
     acc.next_out = g0;
     acc.g_end = (g0 + window < total) ? g0 + window : total;
+    acc.done = false;
     uint64_t p0 = uniform(first_row[w]);
     const uint64_t e_first = uniform(probe_off[p0]);
     bool first = true;
@@ -264,8 +326,8 @@ __global__ __launch_bounds__(kBlock) void window_write_kernel(
     }
     if (MAT && !acc.overflow && (acc.next_out & (kWave - 1)) != 0)  // partial last block
       flush_block<NA>(out, acc.stage, acc.next_out >> 6, (uint32_t)(acc.next_out & (kWave - 1)));
-    if (acc.overflow && lane_id() == 0) atomicOr(overflow, 1u);
   }
+  if (acc.overflow && lane_id() == 0) atomicOr(overflow, 1u);
   block_sum_store(acc.csum, &block_sums[blockIdx.x]);
 }
 

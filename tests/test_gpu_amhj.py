@@ -89,6 +89,25 @@ def test_fib_and_plain_write_identical_arrays(ctx):
         assert np.array_equal(O.sort_tuples(c.tuples()), O.sort_tuples(a.tuples()))
 
 
+def test_window_writer_loops_over_windows(ctx):
+    # 64-row output windows outnumber the writer's waves (its grid is capped), so every wave
+    # writes many windows; the output must equal the unit engine's row for row as a multiset
+    rels = G.chain_relations([1 << 22, 1 << 20, 1 << 20, 1 << 20], 1 << 18, seed=11,
+                             zipf_s=0.5, payload_range=1 << 31)
+    R = [ctx.relation(c, ids) for c, ids in rels]
+    H = [ctx.hash_table(r) for r in R[1:]]
+    want = ctx.execute(R[0], H, materialize=True)
+    assert want.path == "plain" and want.count > 64 * 4 * 65536
+    want_md5 = ctx.md5(want)
+    ctx.free(want)
+    for window in (64, 192):
+        r = ctx.execute(R[0], H, fib=1, out_chunk_rows=window)
+        assert r.path == "fib"
+        assert (r.count, r.checksum) == (want.count, want.checksum)
+        assert ctx.md5(r) == want_md5
+        ctx.free(r)
+
+
 def test_heavy_skew_single_key(ctx):
     # one build key holding every row: runs of 200k rows, results 50 * 200k * 3
     n = 200_000
