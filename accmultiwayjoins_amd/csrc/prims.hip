@@ -208,25 +208,42 @@ __global__ __launch_bounds__(kBlock) void radix_hist_kernel(const uint64_t* __re
   hist[(uint64_t)threadIdx.x * tiles + blockIdx.x] = cnt[threadIdx.x];
 }
 
-// Stable scatter: elements are ranked in (round, wave, lane) order, which is the input order
-// because element (round r, thread t) is key[tile*4096 + r*256 + t].
+// Stable scatter of one 4096-key tile. Wave w owns the tile's keys [w*1024, (w+1)*1024), 64
+// per round, so (wave, round, lane) is the input order. Each wave ranks its keys with a ballot
+// multisplit and a wave-private running count per digit (no block barrier between rounds);
+// the keys are then placed in digit order in LDS and leave in that order, so consecutive
+// lanes write consecutive slots of a bucket's run instead of 8-byte writes all over the
+// output.
 __global__ __launch_bounds__(kBlock) void radix_scatter_kernel(
     const uint64_t* __restrict__ keys, uint64_t* __restrict__ out, uint64_t n, int shift,
     const uint32_t* __restrict__ hist_scanned, uint64_t tiles) {
-  __shared__ uint32_t running[kRadix];
-  __shared__ uint32_t wcnt[kWavesPerBlock][kRadix];
+  constexpr int kPerWave = kSortTile / kWavesPerBlock;
+  __shared__ uint64_t tile[kSortTile];
+  __shared__ uint32_t wrun[kWavesPerBlock][kRadix];  // per-wave digit counts, then offsets
+  __shared__ uint32_t loff[kRadix];                  // first slot of each digit in the tile
+  __shared__ uint32_t goff[kRadix];                  // first output row of each digit
+  __shared__ uint32_t scan_lds[kWavesPerBlock];
   const int tid = threadIdx.x;
   const int w = tid / kWave;
-  running[tid] = hist_scanned[(uint64_t)tid * tiles + blockIdx.x];
+  const int lane = lane_id();
 #pragma unroll
-  for (int i = 0; i < kWavesPerBlock; ++i) wcnt[i][tid] = 0;
+  for (int i = 0; i < kWavesPerBlock; ++i) wrun[i][tid] = 0;
+  goff[tid] = hist_scanned[(uint64_t)tid * tiles + blockIdx.x];
   __syncthreads();
-  const uint64_t base = (uint64_t)blockIdx.x * kSortTile;
+  const uint64_t t0 = (uint64_t)blockIdx.x * kSortTile;
+  const uint64_t base = t0 + (uint64_t)w * kPerWave;
+  uint64_t key[kSortItems];
+  uint32_t rk[kSortItems];
+#pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
-    const uint64_t idx = base + (uint64_t)r * kBlock + tid;
-    const bool valid = idx < n;
-    const uint64_t key = valid ? keys[idx] : 0ull;
-    const uint32_t d = (uint32_t)(key >> shift) & 0xffu;
+    const uint64_t idx = base + (uint64_t)r * kWave + lane;
+    key[r] = idx < n ? keys[idx] : 0ull;
+  }
+  uint32_t* run = wrun[w];
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    const bool valid = base + (uint64_t)r * kWave + lane < n;
+    const uint32_t d = (uint32_t)(key[r] >> shift) & 0xffu;
     uint64_t m = __ballot(valid);
 #pragma unroll
     for (int b = 0; b < 8; ++b) {
@@ -235,24 +252,48 @@ __global__ __launch_bounds__(kBlock) void radix_scatter_kernel(
       m &= bit ? bb : ~bb;
     }
     const uint32_t rank = mbcnt(m);
-    if (valid && rank == 0) wcnt[w][d] = (uint32_t)__popcll(m);
-    __syncthreads();
-    if (valid) {
-      uint32_t pos = running[d] + rank;
-#pragma unroll
-      for (int i = 0; i < kWavesPerBlock; ++i)
-        if (i < w) pos += wcnt[i][d];
-      out[pos] = key;
-    }
-    __syncthreads();
-    uint32_t add = 0;
+    __asm__ __volatile__("" ::: "memory");  // LDS ops of one wave complete in order
+    const uint32_t before = run[d];
+    rk[r] = before + rank;
+    __asm__ __volatile__("" ::: "memory");
+    if (valid && rank == 0) run[d] = before + (uint32_t)__popcll(m);
+    __asm__ __volatile__("" ::: "memory");
+  }
+  __syncthreads();
+  {  // digit tid: tile offset, then each wave's first slot for it
+    uint32_t c[kWavesPerBlock], tot = 0;
 #pragma unroll
     for (int i = 0; i < kWavesPerBlock; ++i) {
-      add += wcnt[i][tid];
-      wcnt[i][tid] = 0;
+      c[i] = wrun[i][tid];
+      tot += c[i];
     }
-    running[tid] += add;
-    __syncthreads();
+    uint32_t all;
+    uint32_t p = block_excl_scan<uint32_t>(tot, &all, scan_lds);
+    loff[tid] = p;
+#pragma unroll
+    for (int i = 0; i < kWavesPerBlock; ++i) {
+      wrun[i][tid] = p;
+      p += c[i];
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    if (base + (uint64_t)r * kWave + lane < n) {
+      const uint32_t d = (uint32_t)(key[r] >> shift) & 0xffu;
+      tile[run[d] + rk[r]] = key[r];
+    }
+  }
+  __syncthreads();
+  const uint32_t tile_n = (uint32_t)(n - t0 < (uint64_t)kSortTile ? n - t0 : (uint64_t)kSortTile);
+#pragma unroll
+  for (int i = 0; i < kSortItems; ++i) {
+    const uint32_t j = (uint32_t)i * kBlock + tid;
+    if (j < tile_n) {
+      const uint64_t k = tile[j];
+      const uint32_t d = (uint32_t)(k >> shift) & 0xffu;
+      out[(uint64_t)goff[d] + (j - loff[d])] = k;
+    }
   }
 }
 
