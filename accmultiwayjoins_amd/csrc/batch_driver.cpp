@@ -1,6 +1,6 @@
 // Host driver of batched AMHJ execution (amj_batch_*): plans of many queries over resident
 // relations and hash tables, executed together by the batch kernels (batch.hip) with two host
-// round trips per batch whatever its size: counts -> one slab for every materialised result
+// round trips per batch whatever its size: counts -> one pooled buffer per materialised result
 // -> writes and checksums.
 #include <algorithm>
 #include <cstring>
@@ -18,7 +18,11 @@ hipError_t launch_batch_totals(const void* qs, uint32_t nq, const uint64_t* base
                                uint64_t* totals, hipStream_t s);
 size_t batch_desc_bytes();
 void batch_fill_desc(void* dst, const Plan& P, const ProbeOut& out, uint64_t unit0,
-                     uint64_t units);
+                     uint64_t units, uint64_t stride);
+hipError_t launch_batch_direct(int L, const void* qs, uint32_t nq, uint64_t unit_begin,
+                               uint64_t total, unsigned long long* rows,
+                               unsigned long long* counts, unsigned long long* csums,
+                               hipStream_t s);
 
 }  // namespace amj
 
@@ -30,6 +34,10 @@ struct amj_batch {
     Plan plan;
     std::vector<uint32_t> attr_ids;
     bool materialize = true;
+    bool direct = false;    // dro: no count pass (output sized from a sample)
+    uint64_t capacity = 0;  // dro: output rows given by the caller (0: from the sample)
+    bool sampled = false;   // dro: the output size came from a sample launch
+    int status = AMJ_OK;    // AMJ_ERR_NOMEM: not admitted (the budget was full before it)
     amj_result_t res{};
     bool taken = false;
   };
@@ -66,6 +74,8 @@ static int batch_add_plan(amj_batch_t* b, Plan& plan, std::vector<uint32_t>& ids
   q.plan = plan;
   q.attr_ids = ids;
   q.materialize = opts ? opts->materialize != 0 : true;
+  q.direct = opts && opts->dro;
+  q.capacity = opts && opts->dro ? opts->dro_capacity_rows : 0;
   b->qs.push_back(q);
   if (id) *id = (uint32_t)(b->qs.size() - 1);
   return AMJ_OK;
@@ -107,6 +117,280 @@ int amj_batch_add_pattern(amj_batch_t* b, const amj_relation_t* edges,
   return batch_add_plan(b, plan, ids, opts, id);
 }
 
+}  // extern "C"
+
+namespace {
+
+// Output buffers of the materialised queries of one phase (rows[k] rows of qs[order[k]]), from
+// the context's pool; all or nothing. Fails with AMJ_ERR_NOMEM, b->needed set, when their bytes
+// exceed the batch budget or the device.
+int alloc_outputs(amj_batch_t* b, const std::vector<uint32_t>& order,
+                  const std::vector<uint64_t>& rows) {
+  amj_ctx_t* ctx = b->ctx;
+  uint64_t bytes = 0;
+  for (size_t k = 0; k < order.size(); ++k)
+    if (b->qs[order[k]].materialize && rows[k] > 0)
+      bytes += rows[k] * (uint64_t)b->qs[order[k]].plan.n_attrs * 4;
+  if (bytes == 0) return AMJ_OK;
+  const uint64_t room = budget_left(ctx, nullptr);
+  if (bytes > (b->budget ? std::min(b->budget, room) : room)) {
+    b->needed = bytes;
+    set_error(ctx, AMJ_ERR_NOMEM,
+              "the batch's results (" + std::to_string(bytes) + " bytes) do not fit the budget");
+    return AMJ_ERR_NOMEM;
+  }
+  for (size_t k = 0; k < order.size(); ++k) {
+    auto& q = b->qs[order[k]];
+    std::memset(&q.res, 0, sizeof(q.res));
+    if (!q.materialize || rows[k] == 0) continue;
+    const int rc = alloc_output(ctx, rows[k], q.plan.n_attrs, &q.res);
+    if (rc != AMJ_OK) {
+      for (size_t j = 0; j <= k; ++j) amj_result_free(ctx, &b->qs[order[j]].res);
+      b->needed = bytes;
+      return rc;
+    }
+  }
+  b->needed += bytes;
+  return AMJ_OK;
+}
+
+// Fill in the result of query q; its output buffer (if any) is already in q.res.
+void set_result(amj_batch::Query& q, uint64_t count, uint64_t checksum) {
+  q.res.count = count;
+  q.res.checksum = checksum;
+  q.res.num_cols = (uint32_t)q.plan.n_attrs;
+  for (int a = 0; a < q.plan.n_attrs; ++a) q.res.attr_ids[a] = q.attr_ids[a];
+  q.res.rows_written = q.res.cols[0] ? count : 0;
+  q.res.passes = 2;
+  q.res.path = AMJ_PATH_BATCH;
+}
+
+ProbeOut output_of(const amj_batch::Query& q, uint64_t capacity) {
+  ProbeOut out{};
+  if (q.res.cols[0]) {
+    for (int a = 0; a < q.plan.n_attrs; ++a) out.cols[a] = q.res.cols[a];
+    out.capacity = capacity;
+  }
+  return out;
+}
+
+// Queries sorted by depth (one launch per group of equal depth); calls f(k, e, L) per group.
+template <typename F>
+int for_depth_groups(amj_batch_t* b, const std::vector<uint32_t>& order, F f) {
+  const uint32_t n = (uint32_t)order.size();
+  for (uint32_t k = 0; k < n;) {
+    uint32_t e = k;
+    const int L = b->qs[order[k]].plan.n_levels;
+    while (e < n && b->qs[order[e]].plan.n_levels == L) ++e;
+    AMJ_TRY(f(k, e, L));
+    k = e;
+  }
+  return AMJ_OK;
+}
+
+std::vector<uint32_t> depth_order(amj_batch_t* b, std::vector<uint32_t> idx) {
+  std::stable_sort(idx.begin(), idx.end(), [&](uint32_t x, uint32_t y) {
+    return b->qs[x].plan.n_levels < b->qs[y].plan.n_levels;
+  });
+  return idx;
+}
+
+// Count pass over every unit (or over a strided sample of units: stride[k] > 1), one scan;
+// returns per-query totals. `desc` receives the descriptors (count-only outputs).
+int batch_counts(amj_batch_t* b, const std::vector<uint32_t>& order,
+                        const std::vector<uint64_t>& units, const std::vector<uint64_t>& stride,
+                        std::vector<uint64_t>* unit0, std::vector<uint64_t>* cnt, void** ddesc_out,
+                        void** counts_out) {
+  amj_ctx_t* ctx = b->ctx;
+  hipStream_t s = ctx->stream;
+  const uint32_t nq = (uint32_t)order.size();
+  unit0->assign(nq, 0);
+  uint64_t total = 0;
+  for (uint32_t k = 0; k < nq; ++k) {
+    (*unit0)[k] = total;
+    total += units[k];
+  }
+  const size_t db = batch_desc_bytes();
+  std::vector<uint8_t> desc(db * nq);
+  for (uint32_t k = 0; k < nq; ++k)
+    batch_fill_desc(desc.data() + db * k, b->qs[order[k]].plan, ProbeOut{}, (*unit0)[k], units[k],
+                    stride[k]);
+  void *ddesc = nullptr, *counts = nullptr, *tmp = nullptr, *totals = nullptr;
+  AMJ_TRY(scratch(ctx, SLOT_MISC0, desc.size(), &ddesc));
+  AMJ_TRY(scratch(ctx, SLOT_UNIT_COUNTS, (total + 1) * 8, &counts));
+  AMJ_TRY(scratch(ctx, SLOT_SCAN_TEMP, scan_temp_elems(total + 1) * 8, &tmp));
+  AMJ_TRY(scratch(ctx, SLOT_MISC3, (size_t)nq * 24, &totals));
+  AMJ_HIP_TRY(ctx, hipMemcpyAsync(ddesc, desc.data(), desc.size(), hipMemcpyHostToDevice, s));
+  AMJ_HIP_TRY(ctx, hipMemsetAsync(counts, 0, (total + 1) * 8, s));
+  AMJ_TRY(for_depth_groups(b, order, [&](uint32_t k, uint32_t e, int L) -> int {
+    const uint64_t gu = (*unit0)[e - 1] + units[e - 1] - (*unit0)[k];
+    AMJ_HIP_TRY(ctx, launch_batch_count(L, (const uint8_t*)ddesc + db * k, e - k, (*unit0)[k],
+                                        gu, (uint64_t*)counts, s));
+    return AMJ_OK;
+  }));
+  AMJ_HIP_TRY(ctx, exclusive_scan_u64((uint64_t*)counts, (uint64_t*)counts, total + 1,
+                                      (uint64_t*)tmp, s));
+  AMJ_HIP_TRY(ctx, launch_batch_totals(ddesc, nq, (const uint64_t*)counts, (uint64_t*)totals, s));
+  cnt->assign(nq, 0);
+  AMJ_HIP_TRY(ctx, hipMemcpyAsync(cnt->data(), totals, nq * 8, hipMemcpyDeviceToHost, s));
+  AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
+  *ddesc_out = ddesc;
+  if (counts_out) *counts_out = counts;
+  return AMJ_OK;
+}
+
+// Exact execution: count pass, one scan, exact outputs, write pass (deterministic row order: every
+// tuple lands where the unit engine puts it).
+int run_exact(amj_batch_t* b, const std::vector<uint32_t>& idx) {
+  amj_ctx_t* ctx = b->ctx;
+  hipStream_t s = ctx->stream;
+  const std::vector<uint32_t> order = depth_order(b, idx);
+  const uint32_t nq = (uint32_t)order.size();
+  std::vector<uint64_t> units(nq), stride(nq, 1), unit0, cnt;
+  for (uint32_t k = 0; k < nq; ++k) units[k] = probe_units(b->qs[order[k]].plan);
+  void *ddesc = nullptr, *counts = nullptr;
+  AMJ_TRY(batch_counts(b, order, units, stride, &unit0, &cnt, &ddesc, &counts));
+  AMJ_TRY(alloc_outputs(b, order, cnt));
+  const size_t db = batch_desc_bytes();
+  std::vector<uint8_t> desc(db * nq);
+  for (uint32_t k = 0; k < nq; ++k) {
+    auto& q = b->qs[order[k]];
+    set_result(q, cnt[k], 0);
+    batch_fill_desc(desc.data() + db * k, q.plan, output_of(q, cnt[k]), unit0[k], units[k], 1);
+  }
+  void* csums = nullptr;
+  AMJ_TRY(scratch(ctx, SLOT_MISC2, (size_t)nq * 8, &csums));
+  AMJ_HIP_TRY(ctx, hipMemcpyAsync(ddesc, desc.data(), desc.size(), hipMemcpyHostToDevice, s));
+  AMJ_HIP_TRY(ctx, hipMemsetAsync(csums, 0, nq * 8, s));
+  AMJ_TRY(for_depth_groups(b, order, [&](uint32_t k, uint32_t e, int L) -> int {
+    const uint64_t gu = unit0[e - 1] + units[e - 1] - unit0[k];
+    AMJ_HIP_TRY(ctx, launch_batch_write(L, (const uint8_t*)ddesc + db * k, e - k, unit0[k], gu,
+                                        (const uint64_t*)counts,
+                                        (unsigned long long*)csums + k, s));
+    return AMJ_OK;
+  }));
+  std::vector<uint64_t> cs(nq);
+  AMJ_HIP_TRY(ctx, hipMemcpyAsync(cs.data(), csums, nq * 8, hipMemcpyDeviceToHost, s));
+  AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
+  for (uint32_t k = 0; k < nq; ++k) b->qs[order[k]].res.checksum = cs[k];
+  return AMJ_OK;
+}
+
+// Direct output (dro): per-query capacities from a strided sample of units, one pass that
+// reserves output rows with atomics. Materialised queries whose results overran their capacity
+// are returned in *redo (their counts and checksums are exact, their rows incomplete).
+constexpr uint64_t kSampleUnits = 1024;  // units sampled per query (all of a smaller query)
+
+int run_direct(amj_batch_t* b, const std::vector<uint32_t>& idx,
+               std::vector<uint32_t>* redo) {
+  amj_ctx_t* ctx = b->ctx;
+  hipStream_t s = ctx->stream;
+  // 1. an output size for every query: exact for a query of at most kSampleUnits units, else
+  //    the count of a strided sample of its units scaled up, with slack
+  std::vector<uint64_t> capq(b->qs.size(), 0);
+  {
+    const std::vector<uint32_t> order = depth_order(b, idx);
+    const uint32_t nq = (uint32_t)order.size();
+    std::vector<uint64_t> units(nq), sunits(nq), stride(nq), unit0, sampled(nq, 0);
+    bool need_sample = false;
+    for (uint32_t k = 0; k < nq; ++k) {
+      units[k] = probe_units(b->qs[order[k]].plan);
+      sunits[k] = std::min(units[k], kSampleUnits);
+      stride[k] = sunits[k] ? units[k] / sunits[k] : 1;
+      need_sample |= b->qs[order[k]].materialize && !b->qs[order[k]].capacity;
+    }
+    void* ddesc = nullptr;
+    if (need_sample)
+      AMJ_TRY(batch_counts(b, order, sunits, stride, &unit0, &sampled, &ddesc, nullptr));
+    for (uint32_t k = 0; k < nq; ++k) {
+      auto& q = b->qs[order[k]];
+      if (!q.materialize) continue;
+      if (q.capacity) {
+        capq[order[k]] = q.capacity;
+      } else if (sunits[k] == units[k]) {
+        capq[order[k]] = sampled[k];
+      } else {
+        const double est = (double)sampled[k] * (double)units[k] / (double)sunits[k];
+        capq[order[k]] = (uint64_t)(est * 1.25) + 65536;
+      }
+      q.sampled = need_sample;
+    }
+  }
+  // 2. admission in submission order: the longest prefix whose outputs fit the room; the
+  //    rest is not run (amj_batch_result reports AMJ_ERR_NOMEM for it)
+  std::vector<uint32_t> admitted;
+  {
+    const uint64_t room = budget_left(ctx, nullptr);
+    const uint64_t lim = b->budget ? std::min(b->budget, room) : room;
+    uint64_t bytes = 0;
+    for (uint32_t i : idx) {
+      const auto& q = b->qs[i];
+      const uint64_t need = q.materialize ? capq[i] * (uint64_t)q.plan.n_attrs * 4 : 0;
+      if (!admitted.empty() && bytes + need > lim) break;
+      if (admitted.empty() && need > lim) {
+        b->needed = need;
+        set_error(ctx, AMJ_ERR_NOMEM, "the batch's results (" + std::to_string(need) +
+                                          " bytes) do not fit the budget");
+        return AMJ_ERR_NOMEM;
+      }
+      bytes += need;
+      admitted.push_back(i);
+    }
+    for (size_t j = admitted.size(); j < idx.size(); ++j) b->qs[idx[j]].status = AMJ_ERR_NOMEM;
+  }
+  // 3. one direct-output pass over the admitted queries
+  const std::vector<uint32_t> order = depth_order(b, admitted);
+  const uint32_t nq = (uint32_t)order.size();
+  std::vector<uint64_t> units(nq), cap(nq), unit0(nq);
+  uint64_t total = 0;
+  for (uint32_t k = 0; k < nq; ++k) {
+    units[k] = probe_units(b->qs[order[k]].plan);
+    cap[k] = capq[order[k]];
+    unit0[k] = total;
+    total += units[k];
+  }
+  AMJ_TRY(alloc_outputs(b, order, cap));
+  const size_t db = batch_desc_bytes();
+  std::vector<uint8_t> desc(db * nq);
+  for (uint32_t k = 0; k < nq; ++k) {
+    const auto& q = b->qs[order[k]];
+    batch_fill_desc(desc.data() + db * k, q.plan, output_of(q, cap[k]), unit0[k], units[k], 1);
+  }
+  void *ddesc = nullptr, *acc = nullptr;
+  AMJ_TRY(scratch(ctx, SLOT_MISC0, desc.size(), &ddesc));
+  AMJ_TRY(scratch(ctx, SLOT_MISC2, (size_t)nq * 24, &acc));
+  unsigned long long* rows = (unsigned long long*)acc;
+  unsigned long long* counts = rows + nq;
+  unsigned long long* csums = rows + 2 * nq;
+  AMJ_HIP_TRY(ctx, hipMemcpyAsync(ddesc, desc.data(), desc.size(), hipMemcpyHostToDevice, s));
+  AMJ_HIP_TRY(ctx, hipMemsetAsync(acc, 0, (size_t)nq * 24, s));
+  AMJ_TRY(for_depth_groups(b, order, [&](uint32_t k, uint32_t e, int L) -> int {
+    const uint64_t gu = unit0[e - 1] + units[e - 1] - unit0[k];
+    AMJ_HIP_TRY(ctx, launch_batch_direct(L, (const uint8_t*)ddesc + db * k, e - k, unit0[k], gu,
+                                         rows + k, counts + k, csums + k, s));
+    return AMJ_OK;
+  }));
+  std::vector<uint64_t> h(3 * (size_t)nq);
+  AMJ_HIP_TRY(ctx, hipMemcpyAsync(h.data(), acc, (size_t)nq * 24, hipMemcpyDeviceToHost, s));
+  AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
+  for (uint32_t k = 0; k < nq; ++k) {
+    auto& q = b->qs[order[k]];
+    const uint64_t cnt = h[nq + k], cs = h[2 * nq + k];
+    if (q.materialize && cnt > cap[k]) {  // rows past the capacity were dropped: run again
+      amj_result_free(ctx, &q.res);
+      redo->push_back(order[k]);
+      continue;
+    }
+    set_result(q, cnt, cs);
+    q.res.passes = q.sampled ? 2 : 1;
+  }
+  return AMJ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int amj_batch_run(amj_batch_t* b) {
   if (!b) return AMJ_ERR_ARG;
   amj_ctx_t* ctx = b->ctx;
@@ -117,132 +401,39 @@ int amj_batch_run(amj_batch_t* b) {
   hipStream_t s = ctx->stream;
   const uint32_t nq = (uint32_t)b->qs.size();
   b->ran = true;
+  b->needed = 0;
   if (nq == 0) return AMJ_OK;
-  // queries grouped by their number of build tables; units numbered consecutively
-  std::vector<uint32_t> order(nq);
-  for (uint32_t i = 0; i < nq; ++i) order[i] = i;
-  std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
-    return b->qs[x].plan.n_levels < b->qs[y].plan.n_levels;
-  });
-  std::vector<uint64_t> unit0(nq), units(nq);
-  uint64_t total = 0;
-  for (uint32_t k = 0; k < nq; ++k) {
-    const Plan& P = b->qs[order[k]].plan;
-    unit0[k] = total;
-    units[k] = probe_units(P);
-    total += units[k];
-  }
-  const size_t db = batch_desc_bytes();
-  std::vector<uint8_t> desc(db * nq);
-  for (uint32_t k = 0; k < nq; ++k)
-    batch_fill_desc(desc.data() + db * k, b->qs[order[k]].plan, ProbeOut{}, unit0[k], units[k]);
-  void *ddesc = nullptr, *counts = nullptr, *tmp = nullptr, *totals = nullptr, *csums = nullptr;
-  AMJ_TRY(scratch(ctx, SLOT_MISC0, desc.size(), &ddesc));
-  AMJ_TRY(scratch(ctx, SLOT_UNIT_COUNTS, (total + 1) * 8, &counts));
-  AMJ_TRY(scratch(ctx, SLOT_SCAN_TEMP, scan_temp_elems(total + 1) * 8, &tmp));
-  AMJ_TRY(scratch(ctx, SLOT_MISC3, (size_t)nq * 16, &totals));
-  csums = (uint64_t*)totals + nq;
+  std::vector<uint32_t> exact, direct;
+  for (uint32_t i = 0; i < nq; ++i) (b->qs[i].direct ? direct : exact).push_back(i);
   AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[2], s));
-  AMJ_HIP_TRY(ctx, hipMemcpyAsync(ddesc, desc.data(), desc.size(), hipMemcpyHostToDevice, s));
-  AMJ_HIP_TRY(ctx, hipMemsetAsync(counts, 0, (total + 1) * 8, s));
-  // count pass: one launch per group of equal depth
-  for (uint32_t k = 0; k < nq;) {
-    uint32_t e = k;
-    const int L = b->qs[order[k]].plan.n_levels;
-    while (e < nq && b->qs[order[e]].plan.n_levels == L) ++e;
-    const uint64_t gu = unit0[e - 1] + units[e - 1] - unit0[k];
-    AMJ_HIP_TRY(ctx, launch_batch_count(L, (const uint8_t*)ddesc + db * k, e - k, unit0[k], gu,
-                                        (uint64_t*)counts, s));
-    k = e;
+  int rc = AMJ_OK;
+  if (!direct.empty()) rc = run_direct(b, direct, &exact);
+  if (rc == AMJ_OK && !exact.empty()) rc = run_exact(b, exact);
+  if (rc != AMJ_OK) {  // nothing is handed out: give back what a phase produced
+    for (auto& q : b->qs)
+      if (q.res.cols[0]) amj_result_free(ctx, &q.res);
+    for (auto& q : b->qs) std::memset(&q.res, 0, sizeof(q.res));
+    b->ran = rc != AMJ_ERR_NOMEM;  // a batch over budget can run again with a larger one
+    return rc;
   }
-  AMJ_HIP_TRY(ctx, exclusive_scan_u64((uint64_t*)counts, (uint64_t*)counts, total + 1,
-                                      (uint64_t*)tmp, s));
-  AMJ_HIP_TRY(ctx, launch_batch_totals(ddesc, nq, (const uint64_t*)counts, (uint64_t*)totals, s));
-  std::vector<uint64_t> cnt(nq);
-  AMJ_HIP_TRY(ctx, hipMemcpyAsync(cnt.data(), totals, nq * 8, hipMemcpyDeviceToHost, s));
-  AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
-  // one slab for every materialised result
-  uint64_t slab_bytes = 0;
-  std::vector<uint64_t> off(nq, 0);
-  for (uint32_t k = 0; k < nq; ++k) {
-    const auto& q = b->qs[order[k]];
-    if (!q.materialize || cnt[k] == 0) continue;
-    off[k] = slab_bytes;
-    slab_bytes += (cnt[k] * (uint64_t)q.plan.n_attrs * 4 + 255) / 256 * 256;
-  }
-  void* slab = nullptr;
-  b->needed = slab_bytes;
-  if (slab_bytes) {
-    const uint64_t room = budget_left(ctx, nullptr);
-    if (slab_bytes > (b->budget ? std::min(b->budget, room) : room)) {
-      set_error(ctx, AMJ_ERR_NOMEM, "the batch's results (" + std::to_string(slab_bytes) +
-                                        " bytes) do not fit the budget");
-      return AMJ_ERR_NOMEM;
-    }
-    amj_result_t tmpres{};
-    AMJ_TRY(alloc_output(ctx, slab_bytes / 4, 1, &tmpres));  // pooled, one column of words
-    slab = tmpres.cols[0];
-    amj_ctx::Slab sl;
-    sl.bytes = (size_t)tmpres.reserved[0] | ((size_t)tmpres.reserved[1] << 32);
-    sl.refs = 0;
-    ctx->slabs[slab] = sl;
-  }
-  for (uint32_t k = 0; k < nq; ++k) {
-    auto& q = b->qs[order[k]];
-    ProbeOut out{};
-    std::memset(&q.res, 0, sizeof(q.res));
-    q.res.count = cnt[k];
-    q.res.num_cols = (uint32_t)q.plan.n_attrs;
-    for (int a = 0; a < q.plan.n_attrs; ++a) q.res.attr_ids[a] = q.attr_ids[a];
-    if (q.materialize && cnt[k] > 0) {
-      int32_t* base = (int32_t*)((uint8_t*)slab + off[k]);
-      for (int a = 0; a < q.plan.n_attrs; ++a) {
-        out.cols[a] = base + (uint64_t)a * cnt[k];
-        q.res.cols[a] = out.cols[a];
-      }
-      out.capacity = cnt[k];
-      q.res.rows_written = cnt[k];
-      q.res.reserved[0] = (uint32_t)((uint64_t)(uintptr_t)slab & 0xffffffffu);
-      q.res.reserved[1] = (uint32_t)((uint64_t)(uintptr_t)slab >> 32);
-      q.res.reserved[2] = 2;
-      ctx->slabs[slab].refs++;
-    }
-    batch_fill_desc(desc.data() + db * k, q.plan, out, unit0[k], units[k]);
-  }
-  AMJ_HIP_TRY(ctx, hipMemcpyAsync(ddesc, desc.data(), desc.size(), hipMemcpyHostToDevice, s));
-  AMJ_HIP_TRY(ctx, hipMemsetAsync(csums, 0, nq * 8, s));
-  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
-  for (uint32_t k = 0; k < nq;) {
-    uint32_t e = k;
-    const int L = b->qs[order[k]].plan.n_levels;
-    while (e < nq && b->qs[order[e]].plan.n_levels == L) ++e;
-    const uint64_t gu = unit0[e - 1] + units[e - 1] - unit0[k];
-    AMJ_HIP_TRY(ctx, launch_batch_write(L, (const uint8_t*)ddesc + db * k, e - k, unit0[k], gu,
-                                        (const uint64_t*)counts,
-                                        (unsigned long long*)csums + k, s));
-    k = e;
-  }
-  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
-  std::vector<uint64_t> cs(nq);
-  AMJ_HIP_TRY(ctx, hipMemcpyAsync(cs.data(), csums, nq * 8, hipMemcpyDeviceToHost, s));
   AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[5], s));
   AMJ_HIP_TRY(ctx, hipEventSynchronize(ctx->ev[5]));
   b->run_ms = elapsed_ms(ctx->ev[2], ctx->ev[5]);
-  const double kms = elapsed_ms(ctx->ev[3], ctx->ev[4]);
-  for (uint32_t k = 0; k < nq; ++k) {
-    auto& r = b->qs[order[k]].res;
-    r.checksum = cs[k];
-    r.probe_ms = b->run_ms;
-    r.probe_kernel_ms = kms;
-    r.probe_kernel_launches = 1;
-    r.passes = 2;
-    r.path = AMJ_PATH_BATCH;
+  for (auto& q : b->qs) {
+    q.res.probe_ms = b->run_ms;
+    q.res.probe_kernel_ms = b->run_ms;
+    q.res.probe_kernel_launches = 1;
   }
   return AMJ_OK;
 }
 
 int amj_batch_result(amj_batch_t* b, uint32_t id, amj_result_t* res) {
   if (!b || !res || id >= b->qs.size()) return AMJ_ERR_ARG;
+  if (b->ran && b->qs[id].status != AMJ_OK) {
+    set_error(b->ctx, b->qs[id].status,
+              "query not run: the batch budget was full before it (run it in a later batch)");
+    return b->qs[id].status;
+  }
   if (!b->ran || b->qs[id].taken) {
     set_error(b->ctx, AMJ_ERR_STATE, "the batch has not run, or this result was taken");
     return AMJ_ERR_STATE;

@@ -10,6 +10,8 @@ namespace amj {
 int make_pattern_plan(amj_ctx_t* ctx, const amj_relation* R, const amj_hash_table* H, int query,
                       Plan* plan, std::vector<uint32_t>* ids) {
   // exec_AMHJ.cu:42 A(0,1),B(1,2),C(0,2) and :59 A(0,1),B(1,2),C(2,3),D(0,3),E(0,2),F(1,3)
+  // (make_plan moves each compare-only table to the first level where both its attributes
+  // are bound, so FOUR checks E(0,2) right after B binds attribute 2)
   static const uint32_t tri[][2] = {{0, 1}, {1, 2}, {0, 2}};
   static const uint32_t four[][2] = {{0, 1}, {1, 2}, {2, 3}, {0, 3}, {0, 2}, {1, 3}};
   const uint32_t(*q)[2] = query == AMJ_QUERY_TRI ? tri : four;
@@ -79,7 +81,8 @@ void amj_ctx_destroy(amj_ctx_t* ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipDeviceSynchronize();
   for (auto& kv : ctx->scratch) dfree(ctx, kv.second.p, kv.second.bytes);
-  for (auto& b : ctx->out_pool) dfree(ctx, b.p, b.bytes);
+  pool_clear(ctx);
+  if (ctx->arena.base) dfree(ctx, ctx->arena.base, ctx->arena.size);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -304,20 +307,9 @@ int amj_execute_pattern(amj_ctx_t* ctx, const amj_relation_t* edges, const amj_h
 int amj_result_free(amj_ctx_t* ctx, amj_result_t* res) {
   if (!ctx || !res) return AMJ_ERR_ARG;
   if (res->reserved[2] == 1 && res->cols[0]) {
-    amj_ctx::Buf b;
-    b.p = res->cols[0];
-    b.bytes = (size_t)res->reserved[0] | ((size_t)res->reserved[1] << 32);
-    ctx->out_pool.push_back(b);
-  } else if (res->reserved[2] == 2) {  // one of the results of a batch slab
-    void* base = (void*)(uintptr_t)((uint64_t)res->reserved[0] | ((uint64_t)res->reserved[1] << 32));
-    auto it = ctx->slabs.find(base);
-    if (it != ctx->slabs.end() && --it->second.refs == 0) {
-      amj_ctx::Buf b;
-      b.p = base;
-      b.bytes = it->second.bytes;
-      ctx->out_pool.push_back(b);
-      ctx->slabs.erase(it);
-    }
+    pool_put(ctx, res->cols[0], (size_t)res->reserved[0] | ((size_t)res->reserved[1] << 32));
+  } else if (res->reserved[2] == 2 && res->cols[0]) {
+    arena_release(ctx, res->cols[0], (uint64_t)res->reserved[0] | ((uint64_t)res->reserved[1] << 32));
   }
   std::memset(res->cols, 0, sizeof(res->cols));
   res->reserved[2] = 0;

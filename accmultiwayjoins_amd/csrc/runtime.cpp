@@ -261,22 +261,128 @@ bool fib_eligible(const Plan& P) {
 }
 
 // Output columns: reuse a parked allocation when it is large enough.
+// Freeing a large buffer and allocating another soon after can cost seconds on this device
+// (the freed memory is cleared before it is handed out again), so released result buffers
+// are parked and reused instead.
+constexpr size_t kPoolCapBuffers = 256;
+
+void pool_put(amj_ctx_t* ctx, void* p, size_t bytes) {
+  if (!p) return;
+  ctx->out_pool.push_back(amj_ctx::Buf{p, bytes});
+  ctx->pool_bytes += bytes;
+  const uint64_t cap_bytes = (uint64_t)ctx->props.totalGlobalMem / 3;
+  while (ctx->pool_bytes > cap_bytes || ctx->out_pool.size() > kPoolCapBuffers) {
+    const amj_ctx::Buf b = ctx->out_pool.front();
+    ctx->out_pool.erase(ctx->out_pool.begin());
+    ctx->pool_bytes -= b.bytes;
+    dfree(ctx, b.p, b.bytes);
+  }
+}
+
+void pool_clear(amj_ctx_t* ctx) {
+  for (auto& b : ctx->out_pool) dfree(ctx, b.p, b.bytes);
+  ctx->out_pool.clear();
+  ctx->pool_bytes = 0;
+}
+
+int arena_reserve(amj_ctx_t* ctx, uint64_t bytes) {
+  if (ctx->arena.base) {
+    set_error(ctx, AMJ_ERR_STATE, "the context already has a result arena");
+    return AMJ_ERR_STATE;
+  }
+  bytes = (bytes + 255) / 256 * 256;
+  void* p = nullptr;
+  AMJ_TRY(dmalloc(ctx, &p, bytes));
+  ctx->arena.base = (char*)p;
+  ctx->arena.size = bytes;
+  ctx->arena.free.clear();
+  ctx->arena.free[0] = bytes;
+  ctx->arena.free_bytes = bytes;
+  return AMJ_OK;
+}
+
+static bool arena_take(amj_ctx_t* ctx, uint64_t bytes, void** out) {
+  auto& A = ctx->arena;
+  bytes = (bytes + 255) / 256 * 256;
+  for (auto it = A.free.begin(); it != A.free.end(); ++it) {
+    if (it->second < bytes) continue;
+    const uint64_t off = it->first, len = it->second;
+    A.free.erase(it);
+    if (len > bytes) A.free[off + bytes] = len - bytes;
+    A.free_bytes -= bytes;
+    *out = A.base + off;
+    return true;
+  }
+  return false;
+}
+
+void arena_release(amj_ctx_t* ctx, void* p, uint64_t bytes) {
+  auto& A = ctx->arena;
+  bytes = (bytes + 255) / 256 * 256;
+  uint64_t off = (uint64_t)((char*)p - A.base), len = bytes;
+  A.free_bytes += bytes;
+  auto next = A.free.lower_bound(off);
+  if (next != A.free.end() && off + len == next->first) {  // merge with the range after
+    len += next->second;
+    next = A.free.erase(next);
+  }
+  if (next != A.free.begin()) {  // merge with the range before
+    auto prev = std::prev(next);
+    if (prev->first + prev->second == off) {
+      prev->second += len;
+      return;
+    }
+  }
+  A.free[off] = len;
+}
+
+uint64_t arena_largest(const amj_ctx_t* ctx) {
+  uint64_t m = 0;
+  for (const auto& kv : ctx->arena.free) m = std::max(m, kv.second);
+  return m;
+}
+
+// A result takes the smallest parked buffer that holds it when that buffer is at most twice
+// its size (plus 64 MiB), so a small result never pins a large buffer; otherwise it is
+// allocated, and only when the device is full is the pool given back first.
 int alloc_output(amj_ctx_t* ctx, uint64_t rows, int n_attrs, amj_result_t* res) {
   const size_t bytes = std::max<uint64_t>(rows, 1) * sizeof(int32_t) * (size_t)n_attrs;
   void* base = nullptr;
   size_t got = 0;
+  if (ctx->arena.base) {
+    if (!arena_take(ctx, bytes, &base)) {
+      set_error(ctx, AMJ_ERR_NOMEM, "result of " + std::to_string(bytes) +
+                                        " bytes does not fit the result arena");
+      return AMJ_ERR_NOMEM;
+    }
+    const uint64_t per_col = rows > 0 ? rows : 1;
+    for (int a = 0; a < n_attrs; ++a) res->cols[a] = (int32_t*)base + (uint64_t)a * per_col;
+    res->reserved[0] = (uint32_t)(bytes & 0xffffffffu);
+    res->reserved[1] = (uint32_t)((uint64_t)bytes >> 32);
+    res->reserved[2] = 2;  // cols[0] is the base of an arena range
+    return AMJ_OK;
+  }
+  size_t best = 0;
+  int pick = -1;
   for (size_t i = 0; i < ctx->out_pool.size(); ++i) {
-    if (ctx->out_pool[i].bytes >= bytes) {
-      base = ctx->out_pool[i].p;
-      got = ctx->out_pool[i].bytes;
-      ctx->out_pool.erase(ctx->out_pool.begin() + i);
-      break;
+    const size_t b = ctx->out_pool[i].bytes;
+    if (b >= bytes && b <= 2 * bytes + (64u << 20) && (pick < 0 || b < best)) {
+      pick = (int)i;
+      best = b;
     }
   }
-  if (!base) {
-    for (auto& b : ctx->out_pool) dfree(ctx, b.p, b.bytes);
-    ctx->out_pool.clear();
-    AMJ_TRY(dmalloc(ctx, &base, bytes));
+  if (pick >= 0) {
+    base = ctx->out_pool[pick].p;
+    got = ctx->out_pool[pick].bytes;
+    ctx->out_pool.erase(ctx->out_pool.begin() + pick);
+    ctx->pool_bytes -= got;
+  } else {
+    int rc = dmalloc(ctx, &base, bytes);
+    if (rc == AMJ_ERR_NOMEM && !ctx->out_pool.empty()) {
+      pool_clear(ctx);
+      rc = dmalloc(ctx, &base, bytes);
+    }
+    if (rc != AMJ_OK) return rc;
     got = bytes;
   }
   const uint64_t per_col = got / sizeof(int32_t) / (size_t)n_attrs;
@@ -301,7 +407,8 @@ int finish_checksum(amj_ctx_t* ctx, const uint64_t* block_sums, uint64_t nb,
 uint64_t budget_left(amj_ctx_t* ctx, const amj_opts_t* o) {
   size_t free_b = 0, total_b = 0;
   if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
-  uint64_t left = free_b;
+  uint64_t left = free_b + ctx->pool_bytes;  // parked buffers are reusable
+  if (ctx->arena.base) left = arena_largest(ctx);
   if (o && o->hbm_budget_bytes) {
     const uint64_t cap = o->hbm_budget_bytes > ctx->allocated ? o->hbm_budget_bytes - ctx->allocated : 0;
     left = std::min<uint64_t>(left, cap);

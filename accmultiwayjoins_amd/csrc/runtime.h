@@ -25,14 +25,17 @@ struct amj_ctx {
     size_t bytes = 0;
   };
   std::map<int, Buf> scratch;
-  // Parked result columns, reused by the next result that fits.
+  // Parked result buffers, reused by later results that fit them (pool_put / alloc_output).
   std::vector<Buf> out_pool;
-  // Result slabs shared by the results of one batch: base -> (bytes, live results).
-  struct Slab {
-    size_t bytes = 0;
-    uint32_t refs = 0;
-  };
-  std::map<void*, Slab> slabs;
+  uint64_t pool_bytes = 0;
+  // Result arena (arena_reserve): one device allocation that every result is carved from,
+  // first fit over free ranges kept in address order and merged on release.
+  struct Arena {
+    char* base = nullptr;
+    uint64_t size = 0;
+    std::map<uint64_t, uint64_t> free;  // offset -> bytes
+    uint64_t free_bytes = 0;
+  } arena;
   // Autotuned options per query shape (table and probe handles, rows, materialise).
   struct Tuned {
     amj_opts_t o;
@@ -96,6 +99,16 @@ enum ScratchSlot {
 
 int dmalloc(amj_ctx_t* ctx, void** p, size_t bytes);
 void dfree(amj_ctx_t* ctx, void* p, size_t bytes);
+// Park a result buffer for reuse (at most 256 buffers and a third of the device, oldest
+// out first).
+void pool_put(amj_ctx_t* ctx, void* p, size_t bytes);
+void pool_clear(amj_ctx_t* ctx);
+// Reserve `bytes` of device memory as the context's result arena (results come only from it
+// afterwards; AMJ_ERR_NOMEM when it is full). Once per context.
+int arena_reserve(amj_ctx_t* ctx, uint64_t bytes);
+void arena_release(amj_ctx_t* ctx, void* p, uint64_t bytes);
+// Largest result the arena can hold now (0 without an arena).
+uint64_t arena_largest(const amj_ctx_t* ctx);
 int scratch(amj_ctx_t* ctx, int slot, size_t bytes, void** out);
 
 // A table as used by one query: the same physical table may be used several times with

@@ -2,8 +2,8 @@
 // requests (NORMAL chains, TRI, FOUR over resident relations) are queued by any host thread and
 // served by one worker thread with continuous batching: every cycle takes all requests that
 // arrived while the previous batch was on the GPU (up to max_batch), runs them as one batch
-// (amj_batch_run), and admits their results against an HBM budget (results held by clients
-// count until they are released). A batch whose results would overrun the budget is split in
+// (amj_batch_run, every query with direct output: no count pass), and admits their results
+// against an HBM budget (results held by clients count until they are released). A batch whose results would overrun the budget is split in
 // halves; a single request that does not fit fails with AMJ_ERR_NOMEM. The plain mode runs the
 // same requests one query per launch (the engines behind amj_execute), for comparison.
 //
@@ -80,7 +80,11 @@ struct amj_server {
   double busy_ms = 0.0;
   std::string last_err;
 
+  // Device bytes a result holds: its whole output buffer (a direct-output buffer is sized
+  // from an estimate and is larger than the rows it holds).
   static uint64_t result_bytes(const amj_result_t& r) {
+    if ((r.reserved[2] == 1 || r.reserved[2] == 2) && r.cols[0])
+      return (uint64_t)r.reserved[0] | ((uint64_t)r.reserved[1] << 32);
     return r.rows_written * (uint64_t)r.num_cols * 4;
   }
 
@@ -116,8 +120,15 @@ struct amj_server {
 
   void run_group(Cycle& c, std::vector<Req> reqs) {
     if (reqs.empty()) return;
+    if (!c.deferred.empty()) {  // nothing overtakes a deferred request: results finish in order
+      c.deferred.insert(c.deferred.end(), reqs.begin(), reqs.end());
+      return;
+    }
     if (plain) {
-      for (auto& q : reqs) run_one(c, q);
+      for (auto& q : reqs) {
+        if (!c.deferred.empty()) c.deferred.push_back(q);  // keep the order behind a deferral
+        else run_one(c, q);
+      }
       return;
     }
     uint64_t need = 0;
@@ -128,7 +139,8 @@ struct amj_server {
       std::vector<Req> b(reqs.begin() + reqs.size() / 2, reqs.end());
       run_group(c, a);
       run_group(c, b);
-    } else if (rc == AMJ_ERR_NOMEM && need > budget) {
+    } else if (rc == AMJ_ERR_NOMEM && (need > budget || c.room >= budget)) {
+      // larger than the budget, or no room even with nothing held: waiting cannot help
       c.out.push_back(failed(reqs[0], rc, "the result (" + std::to_string(need) +
                                               " bytes) exceeds the server's HBM budget"));
     } else if (rc == AMJ_ERR_NOMEM) {
@@ -154,6 +166,12 @@ struct amj_server {
       rc = make_pattern_plan(ctx, rels[q.ids[0]].rel, rels[q.ids[0]].ht, q.query, &plan, &ids);
       if (rc == AMJ_OK) rc = execute_plan(ctx, plan, ids, &o, &res);
     }
+    if (rc == AMJ_ERR_NOMEM && c.room < budget) {
+      // the result may fit once held results are released: wait until nothing is held
+      c.deferred_need = budget;
+      c.deferred.push_back(q);
+      return;
+    }
     ++c.batches;
     if (rc != AMJ_OK) c.out.push_back(failed(q, rc, ctx->last_error));
     else finished(c, q, res);
@@ -164,6 +182,7 @@ struct amj_server {
   int run_batch(Cycle& c, std::vector<Req>& reqs, uint64_t* need) {
     amj_opts_t o;
     amj_default_opts(&o);
+    o.dro = 1;  // one pass per batch: outputs sized from a sample, overruns rerun exactly
     amj_batch_t* b = nullptr;
     amj_batch_create(ctx, &b);
     amj_batch_set_budget(b, c.room);
@@ -201,8 +220,15 @@ struct amj_server {
         continue;
       }
       amj_result_t res;
-      amj_batch_result(b, pid[i], &res);
-      finished(c, planned[i], res);
+      const int r = c.deferred.empty() ? amj_batch_result(b, pid[i], &res) : AMJ_ERR_NOMEM;
+      if (r == AMJ_ERR_NOMEM) {  // not admitted: runs after results are released
+        if (c.deferred.empty()) c.deferred_need = 0;
+        c.deferred.push_back(planned[i]);
+      } else if (r != AMJ_OK) {
+        c.out.push_back(failed(planned[i], r, ctx->last_error));
+      } else {
+        finished(c, planned[i], res);
+      }
     }
     amj_batch_destroy(b);
     return AMJ_OK;
@@ -211,11 +237,12 @@ struct amj_server {
   void loop() {
     (void)hipSetDevice(ctx->device);
     std::unique_lock<std::mutex> lk(mu);
-    uint64_t gate = 0;  // room the request at the head of the queue waits for
+    uint64_t gate = 0;         // room the request at the head of the queue waits for
+    bool wait_release = false;  // the head was deferred: retry once something is released
     for (;;) {
       cv_work.wait(lk, [&] {
         const uint64_t room = budget > in_use ? budget - in_use : 0;
-        return stop || !released.empty() || (!pending.empty() && room >= gate);
+        return stop || !released.empty() || (!pending.empty() && !wait_release && room >= gate);
       });
       if (stop && pending.empty()) return;
       std::vector<amj_result_t> rel;
@@ -249,6 +276,7 @@ struct amj_server {
       // deferred requests go back to the head of the queue, in order
       for (auto it = c.deferred.rbegin(); it != c.deferred.rend(); ++it) pending.push_front(*it);
       gate = c.deferred.empty() ? 0 : c.deferred_need;
+      wait_release = !c.deferred.empty();
       if (stop && !c.deferred.empty()) {  // nobody will release any more: fail them
         for (auto& q : c.deferred) done[q.ticket] = failed(q, AMJ_ERR_NOMEM, "server stopped").d;
         pending.clear();
@@ -269,6 +297,14 @@ int amj_server_create(int device, uint64_t hbm_budget_bytes, uint32_t max_batch,
   auto* s = new amj_server();
   s->ctx = ctx;
   s->budget = hbm_budget_bytes ? hbm_budget_bytes : (uint64_t)ctx->props.totalGlobalMem / 2;
+  // Every result is carved from one reservation of the budget: no device allocation (and no
+  // free, whose memory this device clears before handing it out again) per request.
+  const int rc = arena_reserve(ctx, s->budget);
+  if (rc != AMJ_OK) {
+    delete s;
+    amj_ctx_destroy(ctx);
+    return rc;
+  }
   s->max_batch = max_batch ? max_batch : 256;
   s->plain = plain;
   s->worker = std::thread([s] { s->loop(); });

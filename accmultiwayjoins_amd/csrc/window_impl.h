@@ -223,20 +223,21 @@ struct Chain {
         if ((rb >> 6) != (re >> 6)) flush_block<NA>(out, acc.stage, rb >> 6, (uint32_t)kWave);
       }
     };
-    uint32_t c = c0;
-    for (; e1 - c >= (uint32_t)(kEmitGroup * kWave); c += (uint32_t)(kEmitGroup * kWave)) {
-      int pg[kEmitGroup];
-      int32_t sg[kEmitGroup][NV > 0 ? NV : 1];
+    constexpr uint32_t GW = (uint32_t)(kEmitGroup * kWave);  // results per group
+    using GroupVals = int32_t[kEmitGroup][NV > 0 ? NV : 1];
+    auto locate_group = [&](uint32_t cg, int (&pg)[kEmitGroup], GroupVals& sg) {
       sfor<0, kEmitGroup>([&](auto i) {
-        pg[i] = (int)parent_of_slots(l.excl, l.len, c + (uint32_t)i * kWave);
+        pg[i] = (int)parent_of_slots(l.excl, l.len, cg + (uint32_t)i * kWave);
         if constexpr (NV > 0) {
           const uint32_t row =
-              (uint32_t)__shfl(l.base, pg[i], kWave) + c + (uint32_t)i * kWave + lane_id();
+              (uint32_t)__shfl(l.base, pg[i], kWave) + cg + (uint32_t)i * kWave + lane_id();
           sfor<0, NV>([&](auto k) { sg[i][k] = sfx[(uint64_t)k * sst + row]; });
         }
       });
+    };
+    auto store_group = [&](uint32_t cg, const int (&pg)[kEmitGroup], const GroupVals& sg) {
       sfor<0, kEmitGroup>([&](auto i) {
-        const uint32_t ci = c + (uint32_t)i * kWave;
+        const uint32_t ci = cg + (uint32_t)i * kWave;
         int32_t v[NA];
         sfor<0, NB>([&](auto a) { v[a] = __shfl(l.b[a], pg[i], kWave); });
         if constexpr (NV > 0) sfor<0, NV>([&](auto k) { v[NB + k] = sg[i][k]; });
@@ -247,6 +248,14 @@ struct Chain {
           flush_full_block<NA>(out, acc.stage, (e0 + ci) >> 6);
         }
       });
+    };
+    uint32_t c = c0;
+    const uint32_t n_g = (e1 - c0) / GW;
+    for (uint32_t k = 0; k < n_g; ++k, c += GW) {
+      int pg[kEmitGroup];
+      GroupVals sg;
+      locate_group(c, pg, sg);
+      store_group(c, pg, sg);
     }
     for (; c < e1; c += 2u * (uint32_t)kWave) {
       const uint32_t cb = c + (uint32_t)kWave;

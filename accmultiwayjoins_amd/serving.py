@@ -1,7 +1,8 @@
 """Batched and served execution of many join queries (amj_batch_* and amj_server_* of libamj).
 
 Batch: queries over resident relations / hash tables run together, one count launch and one
-write launch per group of equal depth. Server: a worker thread keeps relations resident and
+write launch per group of equal depth (dro=1 queries: a sample launch and one direct-output
+launch, rows placed by atomic reservation, so their row order is not the unit engine's). Server: a worker thread keeps relations resident and
 serves submitted requests with continuous batching under an HBM budget for the results held
 by clients; `plain=True` serves one query per launch instead (the comparison path).
 """
@@ -20,16 +21,18 @@ class Batch:
         ctx._check(N.lib().amj_batch_create(ctx.h, C.byref(h)))
         self.ctx, self.h = ctx, h
 
-    def add(self, probe: Relation, tables: Sequence[HashTable], materialize=True) -> int:
+    def add(self, probe: Relation, tables: Sequence[HashTable], materialize=True, dro=0,
+            dro_capacity_rows=0) -> int:
         hts = (C.c_void_p * len(tables))(*[t.h.value for t in tables])
-        o = _opts(materialize=materialize)
+        o = _opts(materialize=materialize, dro=dro, dro_capacity_rows=dro_capacity_rows)
         qid = C.c_uint32()
         self.ctx._check(N.lib().amj_batch_add(self.h, probe.h, hts, len(tables), C.byref(o),
                                               C.byref(qid)))
         return qid.value
 
-    def add_pattern(self, edges: Relation, table: HashTable, query="TRI", materialize=True) -> int:
-        o = _opts(materialize=materialize)
+    def add_pattern(self, edges: Relation, table: HashTable, query="TRI", materialize=True,
+                    dro=0, dro_capacity_rows=0) -> int:
+        o = _opts(materialize=materialize, dro=dro, dro_capacity_rows=dro_capacity_rows)
         qid = C.c_uint32()
         self.ctx._check(N.lib().amj_batch_add_pattern(self.h, edges.h, table.h, N.QUERY[query],
                                                       C.byref(o), C.byref(qid)))

@@ -38,6 +38,22 @@ def make_inputs(G, scale, s, seed=42):
     return G.chain_relations([n1, nb, nb, nb], k, seed=seed, zipf_s=s, payload_range=1 << 31)
 
 
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "amhj_chain4_window_writer_pmc.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the writer kernel at s=0 from the committed rocprofv3 PMC summary
+    (FETCH_SIZE x the copy-calibrated read factor + WRITE_SIZE), or None without one."""
+    try:
+        with open(TRAFFIC_PROFILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":", 1)[1].strip())
+    except OSError:
+        pass
+    return None
+
+
 def algorithmic_bytes(rels, count, n_attrs=5):
     """Compulsory HBM bytes of one chain join: every input column read once, every build
     table (key + value + bucket pointers) read once, every result tuple written once."""
@@ -110,7 +126,7 @@ def triangle_leg(ctx, G, scale, steps, warmup):
             "verified_against": "ALFTJ BFS (count + checksum)"}
 
 
-def serving_leg(G, n_queries, max_rows, rate, seed=42):
+def serving_leg(G, n_queries, max_rows, rate, seed=42, budget=200 << 30):
     """Mixed stream (60% chains of 3/4/5 relations, 20% TRI, 20% FOUR) over resident relations:
     chain relations log-uniform in [10^4, max_rows] rows with Zipf(s ~ U[0, 1.2]) build-side
     keys (key range = the build size, so a chain returns about as many tuples as its probe
@@ -139,7 +155,7 @@ def serving_leg(G, n_queries, max_rows, rate, seed=42):
     gaps = [rng.expovariate(rate) for _ in range(n_queries)]
 
     def serve(plain):
-        with Server(0, max_batch=256, plain=plain) as srv:
+        with Server(0, hbm_budget_bytes=budget, max_batch=256, plain=plain) as srv:
             cids = [[srv.add_relation(c, a) for c, a in ch] for ch in chains]
             gids = [srv.add_relation([sd[0], sd[1]], (0, 1)) for sd in graphs]
             tickets, lat, outs, errors = [], [], [], []
@@ -195,7 +211,10 @@ def serving_leg(G, n_queries, max_rows, rate, seed=42):
             "speedup": served["queries_per_s"] / plain["queries_per_s"],
             "workload": f"{n_queries} mixed queries (60% chains of 3/4/5 relations of 1e4.."
                         f"{max_rows} rows, 20% TRI, 20% FOUR on R-MAT 12..16), Poisson "
-                        f"arrivals at {rate} queries/s, results materialised",
+                        f"arrivals at {rate} queries/s, results materialised, "
+                        f"{budget / 2**30:.0f} GiB HBM budget for held results",
+            "engines": "served: continuous batching, one sample + one direct-output launch per "
+                       "depth per batch; plain: one query per launch, count + scan + write",
             "verified": "every result (count + checksum) equal in the served and plain runs"}
 
 
@@ -355,7 +374,8 @@ def main():
                    "scale": args.scale, "skews": skews, "ws": args.ws, "dro": args.dro,
                    "fib": args.fib, "per_skew": per_skew},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
+                     "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT)},
         "cpu_baseline": cpu,
         "triangle": tri,
         "serving": None,

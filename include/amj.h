@@ -187,11 +187,19 @@ void amj_autotune_forget(amj_ctx_t* ctx);
  * resident relations and hash tables are queued, then run together: one count launch and one
  * write launch per group of equal depth, one scan, two host round trips per batch. Each result
  * equals the one amj_execute gives for the same query (count, checksum, tuples in the same
- * order). Results are taken with amj_batch_result and released with amj_result_free. */
+ * order). Queries added with opts->dro = 1 skip the count pass instead (the direct result
+ * output of AMHJ_DRO, Joins/AMHJ_DRO.cuh:18-707, applied to the whole batch): each output is
+ * sized from a strided sample of the query's probe units (or from opts->dro_capacity_rows),
+ * one launch reserves rows with atomics, and a query whose results overrun its size is run
+ * again exactly; counts and checksums are the same, the row order is not. dro queries are
+ * admitted in the order they were added while their outputs fit the budget: the rest is not
+ * run and amj_batch_result returns AMJ_ERR_NOMEM for it. Results are taken with
+ * amj_batch_result and released with amj_result_free. */
 int amj_batch_create(amj_ctx_t* ctx, amj_batch_t** out);
 void amj_batch_destroy(amj_batch_t* batch);
 /* Queue a NORMAL query (the arguments of amj_execute); *id numbers it within the batch.
- * Only opts->materialize is used (the batch engine is the count / write unit engine). */
+ * opts->materialize, opts->dro and opts->dro_capacity_rows are used (the batch engine is the
+ * unit engine; ws, fib and ooc do not apply). */
 int amj_batch_add(amj_batch_t* batch, const amj_relation_t* probe, amj_hash_table_t* const* hts,
                   uint32_t n_hts, const amj_opts_t* opts, uint32_t* id);
 /* Queue a TRI / FOUR query over an edge relation and its table (key column 0, sort_vals=1). */
@@ -212,9 +220,12 @@ int amj_batch_info(const amj_batch_t* batch, uint32_t* queries, double* run_ms,
 /* No counterpart in the reference. A server owns a context and a worker thread: relations are
  * registered once (resident, hashed on column 0 with value-sorted runs), requests from any host
  * thread are served with continuous batching (each cycle runs, as one batch, every request that
- * arrived while the previous batch was on the GPU, up to max_batch), and results are admitted
- * against hbm_budget_bytes (bytes of results held by clients until amj_server_release; 0 = half
- * the device). plain = 1 serves one query per launch instead (the comparison path). */
+ * arrived while the previous batch was on the GPU, up to max_batch, as dro batch queries: one
+ * sample launch and one direct-output launch per depth), and results are admitted against
+ * hbm_budget_bytes (bytes of results held by clients until amj_server_release; 0 = half the
+ * device; a result counts with its whole output buffer, which direct output sizes from an
+ * estimate). plain = 1 serves one query per launch instead (amj_execute's default engine, the
+ * comparison path). */
 typedef struct amj_server amj_server_t;
 int amj_server_create(int device, uint64_t hbm_budget_bytes, uint32_t max_batch, int plain,
                       amj_server_t** out);

@@ -68,6 +68,69 @@ def test_batch_equals_individual_queries(ctx):
     assert (want[-3].count, want[-3].checksum) == (n3, c3)
 
 
+def _sorted(cols):
+    t = np.stack(cols, axis=1) if cols else np.zeros((0, 0), np.int32)
+    return t[np.lexsort(t.T[::-1])] if len(t) else t
+
+
+@pytest.mark.parametrize("capacity", [0, 64], ids=["sampled", "overrun-rerun"])
+def test_direct_output_batch_equals_individual_queries(ctx, capacity):
+    # dro=1 queries: no count pass; capacity 64 overruns every larger result, which is then
+    # run again exactly. Same count, checksum and multiset of tuples as each query alone.
+    chains, (src, dst) = _pool()
+    R = [[ctx.relation(c, ids) for c, ids in ch] for ch in chains]
+    H = [[ctx.hash_table(r) for r in rs[1:]] for rs in R]
+    E = ctx.relation([src, dst], [0, 1])
+    HE = ctx.hash_table(E, 0, sort_vals=True)
+    queries = [("chain", k, mat) for k in range(len(R)) for mat in (True, False)]
+    queries += [("TRI", None, True), ("FOUR", None, True), ("FOUR", None, False)]
+    want = []
+    for kind, k, mat in queries:
+        if kind == "chain":
+            want.append(ctx.execute(R[k][0], H[k], fetch=True, materialize=mat))
+        else:
+            want.append(ctx.execute_pattern(E, HE, kind, fetch=True, materialize=mat))
+    b = Batch(ctx)
+    ids = []
+    for kind, k, mat in queries:
+        if kind == "chain":
+            ids.append(b.add(R[k][0], H[k], materialize=mat, dro=1, dro_capacity_rows=capacity))
+        else:
+            ids.append(b.add_pattern(E, HE, kind, materialize=mat, dro=1,
+                                     dro_capacity_rows=capacity))
+    b.run()
+    for (kind, k, mat), qid, w in zip(queries, ids, want):
+        r = b.result(qid, fetch=True)
+        assert (r.count, r.checksum) == (w.count, w.checksum), (kind, k, mat)
+        if mat:
+            assert r.rows_written == w.count
+            assert np.array_equal(_sorted(r.cols), _sorted(w.cols)), (kind, k)
+    b.close()
+    n4, c4 = O.four_cliques(src, dst)
+    assert (want[-1].count, want[-1].checksum) == (n4, c4)
+
+
+def test_direct_output_batch_admits_a_prefix(ctx):
+    # with room for one result, the first dro query runs and the later ones are reported
+    # as not admitted (the server runs them in a later batch)
+    chains, _g = _pool()
+    R = [ctx.relation(c, ids) for c, ids in chains[1]]
+    H = [ctx.hash_table(r) for r in R[1:]]
+    want = ctx.execute(R[0], H, materialize=True)
+    need = want.count * len(want.attr_ids) * 4
+    ctx.free(want)
+    b = Batch(ctx)
+    ids = [b.add(R[0], H, dro=1) for _ in range(3)]
+    b.set_budget(need + need // 2)
+    b.run()
+    r = b.result(ids[0], fetch=True)
+    assert (r.count, r.checksum) == (want.count, want.checksum)
+    for i in ids[1:]:
+        with pytest.raises(AmjError, match="not run"):
+            b.result(i)
+    b.close()
+
+
 def _server_pool(srv):
     chains, (src, dst) = _pool()
     ids = []
@@ -117,9 +180,11 @@ def test_server_same_alone_and_in_mix(plain):
             assert st["batches"] < st["served"]  # requests were batched
 
 
-def test_server_threads_and_budget_splits():
-    # a budget that holds only a few results at once forces batch splits
-    with Server(0, hbm_budget_bytes=16 << 20, max_batch=64) as srv:
+@pytest.mark.parametrize("plain", [False, True], ids=["batched", "plain"])
+def test_server_threads_and_budget_splits(plain):
+    # a budget that holds only a few results at once forces batch splits (batched) or waits
+    # for releases (plain); every request still gets its result
+    with Server(0, hbm_budget_bytes=16 << 20, max_batch=64, plain=plain) as srv:
         reqs, _chains, _g = _server_pool(srv)
         alone = [_key(srv.wait(_submit(srv, q), fetch=True)) for q in reqs]
         results = {}
