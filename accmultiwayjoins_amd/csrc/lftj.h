@@ -59,6 +59,14 @@ struct LOut {
 };
 hipError_t launch_lftj_write(const LPlan& P, const uint64_t* unit_base, const LOut& cols,
                              bool hash, uint64_t* block_sums, uint64_t* n_blocks, hipStream_t s);
+// Counts of `units` units k * stride + offset (an evenly strided sample when stride > 1).
+hipError_t launch_lftj_count(const LPlan& P, uint64_t* counts, hipStream_t s, uint64_t units,
+                             uint64_t stride, uint64_t offset);
+// Direct output: one pass, rows reserved with atomics into *rows (zeroed by the caller); rows at
+// or past `capacity` are dropped, so *rows > capacity means the pass must run again.
+hipError_t launch_lftj_direct(const LPlan& P, const LOut& cols, uint64_t capacity, bool hash,
+                              unsigned long long* rows, uint64_t* block_sums, uint64_t* n_blocks,
+                              hipStream_t s);
 
 // Level-0 arrays of a relation readable from the host.
 struct HostView {

@@ -37,9 +37,97 @@ __device__ __forceinline__ void level0(const LPlan& P, uint64_t unit, LLv<A>& l)
   finish(l);
 }
 
-template <int A, int i, int MODE>
-__device__ __forceinline__ void descend(const LPlan& P, const LLv<A>& par,
-                                        const LOut& cols, LAcc& acc) {
+// What happens to the candidates that pass the last attribute (`ok` per lane, bindings in b):
+// counted, written at their scanned positions, or gathered for direct output.
+struct CountSink {
+  uint64_t count = 0;
+  template <int A>
+  __device__ __forceinline__ void operator()(bool ok, const int32_t (&)[A]) {
+    count += (uint64_t)__popcll(__ballot(ok));
+  }
+};
+
+template <int MODE>
+struct WriteSink {
+  LOut cols;
+  uint64_t out_base;  // absolute row of the next result (uniform)
+  uint64_t csum;      // per-lane checksum partial
+  template <int A>
+  __device__ __forceinline__ void operator()(bool ok, const int32_t (&b)[A]) {
+    const uint64_t m = __ballot(ok);
+    if (ok) {
+      if (mode_hash<MODE>()) {
+        int32_t w[kMaxAttrs];
+        sfor<0, kMaxAttrs>([&](auto a) { w[a] = (a < A) ? b[(a < A) ? (int)a : 0] : 0; });
+        csum += hash_tuple(w, A);
+      }
+      if (mode_mat<MODE>()) {
+        const uint64_t pos = out_base + mbcnt(m);
+        sfor<0, A>([&](auto a) { cols.c[a][pos] = b[a]; });
+      }
+    }
+    out_base += (uint64_t)__popcll(m);
+  }
+};
+
+// Direct output: results gather in a per-wave LDS buffer of kLftjDirectRows rows and leave
+// with one atomic reservation per full buffer; rows at or past `capacity` are not written (the
+// reserved total then tells the host to run again with an exact allocation).
+constexpr uint32_t kLftjDirectRows = 256;
+typedef __attribute__((address_space(3))) int32_t lds_i32;
+
+template <bool HASH>
+struct DirectSink {
+  LOut cols;
+  uint64_t capacity;
+  unsigned long long* rows;  // reserved rows of the whole launch
+  lds_i32* buf;              // attribute a of row i at buf[a * kLftjDirectRows + i]
+  uint32_t fill;             // rows held (uniform)
+  int n_attrs;
+  uint64_t csum;
+
+  __device__ __forceinline__ void flush() {
+    if (fill == 0) return;
+    unsigned long long r0 = 0;
+    if (lane_id() == 0)
+      r0 = __hip_atomic_fetch_add(rows, (unsigned long long)fill, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t base = uniform((uint64_t)r0);
+    __asm__ __volatile__("" ::: "memory");  // LDS ops of one wave complete in order
+    for (uint32_t i = (uint32_t)lane_id(); i < fill; i += kWave) {
+      const uint64_t pos = base + i;
+      if (pos < capacity)
+        sfor<0, kMaxLftjAttrs>([&](auto a) {
+          if ((int)a < n_attrs) cols.c[a][pos] = buf[a * kLftjDirectRows + i];
+        });
+    }
+    __asm__ __volatile__("" ::: "memory");
+    fill = 0;
+  }
+
+  template <int A>
+  __device__ __forceinline__ void operator()(bool ok, const int32_t (&b)[A]) {
+    const uint64_t m = __ballot(ok);
+    const uint32_t n = (uint32_t)__popcll(m);
+    if (n == 0) return;
+    if (HASH && ok) {
+      int32_t w[kMaxAttrs];
+      sfor<0, kMaxAttrs>([&](auto a) { w[a] = (a < A) ? b[(a < A) ? (int)a : 0] : 0; });
+      csum += hash_tuple(w, A);
+    }
+    if (fill + n > kLftjDirectRows) flush();
+    __asm__ __volatile__("" ::: "memory");
+    if (ok) {
+      const uint32_t slot = fill + mbcnt(m);
+      sfor<0, A>([&](auto a) { buf[a * kLftjDirectRows + slot] = b[a]; });
+    }
+    __asm__ __volatile__("" ::: "memory");
+    fill += n;
+  }
+};
+
+template <int A, int i, typename Sink>
+__device__ __forceinline__ void descend(const LPlan& P, const LLv<A>& par, Sink& sink) {
   for (uint32_t c = 0; c < par.T; c += (uint32_t)kWave) {
     const uint32_t g = c + (uint32_t)lane_id();
     const bool valid = g < par.T;
@@ -58,40 +146,28 @@ __device__ __forceinline__ void descend(const LPlan& P, const LLv<A>& par,
     bool ok = valid && check_and_open(P, i, v, drel, drel_root, ch);
     ch.b[i] = v;
     if constexpr (i == A - 1) {
-      const uint64_t m = __ballot(ok);
-      const uint32_t n = (uint32_t)__popcll(m);
-      if (MODE != MODE_COUNT && ok) {
-        if (mode_hash<MODE>()) {
-          int32_t w[kMaxAttrs];
-          sfor<0, kMaxAttrs>([&](auto a) { w[a] = (a < A) ? ch.b[(a < A) ? (int)a : 0] : 0; });
-          acc.csum += hash_tuple(w, A);
-        }
-        if (mode_mat<MODE>()) {
-          const uint64_t pos = acc.out_base + mbcnt(m);
-          sfor<0, A>([&](auto a) { cols.c[a][pos] = ch.b[a]; });
-        }
-      }
-      acc.count += n;
-      acc.out_base += n;
+      sink(ok, ch.b);
     } else {
       choose_driver(P, i + 1, ok, ch);
       finish(ch);
-      if (ch.T) descend<A, i + 1, MODE>(P, ch, cols, acc);
+      if (ch.T) descend<A, i + 1>(P, ch, sink);
     }
   }
 }
 
+// Count pass over units k * stride + offset (stride > 1: an evenly strided sample).
 template <int A>
 __global__ __launch_bounds__(kBlock) void lftj_count_kernel(const LPlan P,
                                                             uint64_t* __restrict__ counts,
-                                                            uint64_t units) {
-  const uint64_t unit = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
-  if (unit >= units) return;  // no barriers in this kernel
+                                                            uint64_t units, uint64_t stride,
+                                                            uint64_t offset) {
+  const uint64_t k = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+  if (k >= units) return;  // no barriers in this kernel
   LLv<A> l0;
-  level0(P, unit, l0);
-  LAcc acc{0, 0, 0};
-  if (l0.T) descend<A, 1, MODE_COUNT>(P, l0, LOut{}, acc);
-  if (lane_id() == 0) counts[unit] = acc.count;
+  level0(P, k * stride + offset, l0);
+  CountSink sink;
+  if (l0.T) descend<A, 1>(P, l0, sink);
+  if (lane_id() == 0) counts[k] = sink.count;
 }
 
 template <int A, int MODE>
@@ -101,28 +177,72 @@ __global__ __launch_bounds__(kBlock) void lftj_write_kernel(const LPlan P,
                                                             uint64_t* __restrict__ block_sums,
                                                             uint64_t units) {
   const uint64_t unit = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
-  LAcc acc{0, 0, 0};
+  WriteSink<MODE> sink{cols, 0, 0};
   if (unit < units) {
     LLv<A> l0;
     level0(P, unit, l0);
-    acc.out_base = uniform(base[unit]);
-    if (l0.T) descend<A, 1, MODE>(P, l0, cols, acc);
+    sink.out_base = uniform(base[unit]);
+    if (l0.T) descend<A, 1>(P, l0, sink);
   }
-  block_sum_store(acc.csum, &block_sums[blockIdx.x]);
+  block_sum_store(sink.csum, &block_sums[blockIdx.x]);
+}
+
+template <int A, bool HASH>
+__global__ __launch_bounds__(kBlock) void lftj_direct_kernel(const LPlan P, const LOut cols,
+                                                             uint64_t capacity,
+                                                             unsigned long long* __restrict__ rows,
+                                                             uint64_t* __restrict__ block_sums,
+                                                             uint64_t units) {
+  __shared__ int32_t stage_mem[kWavesPerBlock * A * kLftjDirectRows];
+  const uint64_t unit = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+  DirectSink<HASH> sink{cols, capacity, rows,
+                        (lds_i32*)&stage_mem[(threadIdx.x / kWave) * A * kLftjDirectRows], 0u,
+                        A, 0};
+  if (unit < units) {
+    LLv<A> l0;
+    level0(P, unit, l0);
+    if (l0.T) descend<A, 1>(P, l0, sink);
+    sink.flush();
+  }
+  block_sum_store(sink.csum, &block_sums[blockIdx.x]);
 }
 
 uint64_t lftj_units(const LPlan& P) { return (P.roots + kWave - 1) / kWave; }
 
-hipError_t launch_lftj_count(const LPlan& P, uint64_t* counts, hipStream_t s) {
-  const uint64_t units = lftj_units(P);
+hipError_t launch_lftj_count(const LPlan& P, uint64_t* counts, hipStream_t s, uint64_t units,
+                             uint64_t stride, uint64_t offset) {
   if (units == 0) return hipSuccess;
   const dim3 grid((unsigned)((units + kWavesPerBlock - 1) / kWavesPerBlock));
   switch (P.n_attrs) {
-    case 2: hipLaunchKernelGGL(lftj_count_kernel<2>, grid, dim3(kBlock), 0, s, P, counts, units); break;
-    case 3: hipLaunchKernelGGL(lftj_count_kernel<3>, grid, dim3(kBlock), 0, s, P, counts, units); break;
-    case 4: hipLaunchKernelGGL(lftj_count_kernel<4>, grid, dim3(kBlock), 0, s, P, counts, units); break;
+    case 2: hipLaunchKernelGGL(lftj_count_kernel<2>, grid, dim3(kBlock), 0, s, P, counts, units, stride, offset); break;
+    case 3: hipLaunchKernelGGL(lftj_count_kernel<3>, grid, dim3(kBlock), 0, s, P, counts, units, stride, offset); break;
+    case 4: hipLaunchKernelGGL(lftj_count_kernel<4>, grid, dim3(kBlock), 0, s, P, counts, units, stride, offset); break;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t launch_lftj_count(const LPlan& P, uint64_t* counts, hipStream_t s) {
+  return launch_lftj_count(P, counts, s, lftj_units(P), 1, 0);
+}
+
+hipError_t launch_lftj_direct(const LPlan& P, const LOut& cols, uint64_t capacity, bool hash,
+                              unsigned long long* rows, uint64_t* block_sums,
+                              uint64_t* n_blocks, hipStream_t s) {
+  const uint64_t units = lftj_units(P);
+  const uint64_t blocks = units == 0 ? 1 : (units + kWavesPerBlock - 1) / kWavesPerBlock;
+  *n_blocks = blocks;
+  const dim3 grid((unsigned)blocks);
+#define LD(AV, H)                                                                             \
+  hipLaunchKernelGGL((lftj_direct_kernel<AV, H>), grid, dim3(kBlock), 0, s, P, cols, capacity, \
+                     rows, block_sums, units)
+  switch (P.n_attrs) {
+    case 2: if (hash) LD(2, true); else LD(2, false); break;
+    case 3: if (hash) LD(3, true); else LD(3, false); break;
+    case 4: if (hash) LD(4, true); else LD(4, false); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef LD
   return hipGetLastError();
 }
 

@@ -91,7 +91,91 @@ static int make_lplan(amj_ctx_t* ctx, const std::vector<LUse>& uses, const uint3
   return AMJ_OK;
 }
 
+#ifndef AMJ_LFTJ_SAMPLE_UNITS
+#define AMJ_LFTJ_SAMPLE_UNITS 1024
+#endif
+constexpr uint64_t kLftjSampleUnits = AMJ_LFTJ_SAMPLE_UNITS;
+
+// DFS with direct output (opts.dro, materialising): the output is sized from the counts of an
+// evenly strided sample of at most kLftjSampleUnits units (x1.5 + 64 Ki rows; exact when every unit is
+// sampled, or opts.dro_capacity_rows), one pass writes through per-wave LDS buffers with one
+// atomic reservation per 256 rows, and a pass that overran its output runs once more into an
+// exact one. Count and checksum are those of the count/write engine; the row order is not.
+static int run_lftj_direct(amj_ctx_t* ctx, const LPlan& P, const amj_opts_t* o,
+                           amj_result_t* res) {
+  hipStream_t s = ctx->stream;
+  const uint64_t units = lftj_units(P);
+  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[2], s));
+  uint64_t capacity = o->dro_capacity_rows;
+  uint32_t passes = 0;
+  if (capacity == 0 && units > 0) {
+    // the sample's units sit mid-stride: unit 0 holds the smallest roots, on skewed graphs the
+    // heaviest ones, and one heavy unit would set the sample launch's length by itself
+    const uint64_t su = std::min<uint64_t>(units, kLftjSampleUnits), stride = units / su;
+    void *counts = nullptr, *tmp = nullptr, *sum = nullptr;
+    AMJ_TRY(scratch(ctx, SLOT_UNIT_COUNTS, (su + 1) * 8, &counts));
+    AMJ_TRY(scratch(ctx, SLOT_SCAN_TEMP, scan_temp_elems(su + 1) * 8, &tmp));
+    AMJ_TRY(scratch(ctx, SLOT_SUM, 64, &sum));
+    AMJ_HIP_TRY(ctx, launch_lftj_count(P, (uint64_t*)counts, s, su, stride, stride / 2));
+    AMJ_HIP_TRY(ctx, reduce_sum_u64((const uint64_t*)counts, su, (uint64_t*)sum, (uint64_t*)tmp, s));
+    uint64_t sampled = 0;
+    AMJ_HIP_TRY(ctx, hipMemcpyAsync(&sampled, sum, 8, hipMemcpyDeviceToHost, s));
+    AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
+    capacity = su == units ? sampled
+                           : (uint64_t)((double)sampled * (double)units / (double)su * 1.5) + 65536;
+    ++passes;
+  }
+  const uint64_t nb = units == 0 ? 1 : (units + kWavesPerBlock - 1) / kWavesPerBlock;
+  void *bsum = nullptr, *ctl = nullptr;
+  AMJ_TRY(scratch(ctx, SLOT_BLOCK_SUMS, nb * 8, &bsum));
+  AMJ_TRY(scratch(ctx, SLOT_FLAG, 256, &ctl));
+  const bool hash = o->checksum != 0;
+  uint64_t rows = 0, nb_used = 0;
+  double kms = 0.0;
+  for (int attempt = 0; attempt < 2; ++attempt) {
+    LOut out{};
+    if (capacity > 0) {
+      if (capacity * (uint64_t)P.n_attrs * 4 > budget_left(ctx, o)) {
+        set_error(ctx, AMJ_ERR_NOMEM, "result of " + std::to_string(capacity) +
+                                          " tuples does not fit the HBM budget (use "
+                                          "materialize=0)");
+        return AMJ_ERR_NOMEM;
+      }
+      AMJ_TRY(alloc_output(ctx, capacity, P.n_attrs, res));
+      for (int a = 0; a < P.n_attrs; ++a) out.c[a] = res->cols[a];
+    }
+    AMJ_HIP_TRY(ctx, hipMemsetAsync(ctl, 0, 8, s));
+    AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
+    AMJ_HIP_TRY(ctx, launch_lftj_direct(P, out, capacity, hash, (unsigned long long*)ctl,
+                                        (uint64_t*)bsum, &nb_used, s));
+    AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
+    AMJ_HIP_TRY(ctx, hipMemcpyAsync(&rows, ctl, 8, hipMemcpyDeviceToHost, s));
+    AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
+    kms += elapsed_ms(ctx->ev[3], ctx->ev[4]);
+    ++passes;
+    if (rows <= capacity) break;
+    amj_result_free(ctx, res);  // overran: the pass counted every result, run again exactly
+    capacity = rows;
+  }
+  if (rows > capacity) {
+    set_error(ctx, AMJ_ERR_STATE, "direct output overran an exact allocation");
+    return AMJ_ERR_STATE;
+  }
+  if (hash) AMJ_TRY(finish_checksum(ctx, (const uint64_t*)bsum, nb_used, &res->checksum));
+  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[5], s));
+  AMJ_HIP_TRY(ctx, hipEventSynchronize(ctx->ev[5]));
+  res->count = rows;
+  res->rows_written = rows;
+  res->probe_ms = elapsed_ms(ctx->ev[2], ctx->ev[5]);
+  res->probe_kernel_ms = kms;
+  res->probe_kernel_launches = passes;
+  res->passes = passes;
+  res->path = AMJ_PATH_LFTJ_DFS;
+  return AMJ_OK;
+}
+
 static int run_lftj_dfs(amj_ctx_t* ctx, const LPlan& P, const amj_opts_t* o, amj_result_t* res) {
+  if (o->dro && o->materialize) return run_lftj_direct(ctx, P, o, res);
   hipStream_t s = ctx->stream;
   const uint64_t units = lftj_units(P);
   void *counts = nullptr, *tmp = nullptr, *bsum = nullptr;

@@ -73,14 +73,15 @@ typedef struct {
 typedef struct {
   int ooc;                      /* stream the probe relation in slices (ooc.cuh:166-306) */
   int ws;                       /* work sharing between waves */
-  int dro;                      /* direct result output (single pass, no count pass) */
+  int dro;                      /* direct result output (single pass, no count pass): AMHJ,
+                                   ALFTJ DFS and batches; row order then depends on timing */
   int fib;                      /* frequent-item buffer (subtree counts of join values) */
   int materialize;              /* 1: write result columns; 0: count + checksum only */
   int checksum;                 /* 1: compute the result checksum (always on when
                                    materialize = 0) */
   uint64_t hbm_budget_bytes;    /* 0: use what the device has free */
   uint64_t ooc_slice_rows;      /* 0: derive from the budget */
-  uint64_t out_chunk_rows;      /* dro: rows per output chunk (0: default) */
+  uint64_t out_chunk_rows;      /* fib: rows per output window (0: 4,096) */
   uint64_t dro_capacity_rows;   /* dro: initial output allocation (0: sampled estimate) */
   uint32_t ws_split_threshold;  /* ws: split runs longer than this (0: default) */
   int fib_suffix;               /* fib: 0 size-based suffix tables, 1 none, 2 deepest that fits */

@@ -207,3 +207,19 @@ def test_out_of_core_multiway_kat(ctx):
         t.offload()
     r = ctx.lftj(tries, [0, 1, 2], "DFS", ooc_slice_rows=1, materialize=False)
     assert (r.count, r.checksum) == (5, O.checksum(KAT_TRIANGLE))
+
+
+@pytest.mark.parametrize("query", ["TRI", "FOUR"])
+@pytest.mark.parametrize("cap", [0, 16], ids=["sampled", "overrun-rerun"])
+def test_dfs_direct_output_same_multiset(ctx, query, cap):
+    # dro=1: one pass through per-wave buffers and atomic reservations; capacity 16 overruns
+    # and runs again into an exact output. Same count, checksum and multiset of tuples.
+    s, d = G.rmat_oriented(11, 16, seed=3)
+    want = ctx.lftj_pattern(s, d, query, "DFS", fetch=True)
+    r = ctx.lftj_pattern(s, d, query, "DFS", fetch=True, dro=1, dro_capacity_rows=cap)
+    assert r.count > 16
+    assert (r.count, r.checksum) == (want.count, want.checksum)
+    assert r.rows_written == r.count
+    assert _tuples(r) == _tuples(want)
+    if cap:
+        assert r.passes == 2  # the first pass overran its output
