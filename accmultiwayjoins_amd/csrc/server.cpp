@@ -2,8 +2,9 @@
 // requests (NORMAL chains, TRI, FOUR over resident relations) are queued by any host thread and
 // served by one worker thread with continuous batching: every cycle takes all requests that
 // arrived while the previous batch was on the GPU (up to max_batch), runs them as one batch
-// (amj_batch_run, every query with direct output: no count pass), and admits their results
-// against an HBM budget (results held by clients count until they are released). A batch whose results would overrun the budget is split in
+// (amj_batch_run, every query with direct output: no count pass; a query seen before is
+// sized from its last exact count, a new one from a sample), and admits their results against
+// an HBM budget (results held by clients count until they are released). A batch whose results would overrun the budget is split in
 // halves; a single request that does not fit fails with AMJ_ERR_NOMEM. The plain mode runs the
 // same requests one query per launch (the engines behind amj_execute), for comparison.
 //
@@ -71,6 +72,15 @@ struct amj_server {
     uint64_t ticket;
     Done d;
   };
+  // Exact result size of every (query, relation ids) served so far (under gpu): relations are
+  // immutable while registered, so a repeated query's output is sized exactly and its batch
+  // needs no sample launch for it.
+  std::map<std::vector<uint32_t>, uint64_t> result_rows;
+  static std::vector<uint32_t> shape(const Req& q) {
+    std::vector<uint32_t> k{(uint32_t)q.query};
+    k.insert(k.end(), q.ids.begin(), q.ids.end());
+    return k;
+  }
   std::deque<Req> pending;             // under mu
   std::map<uint64_t, Done> done;       // under mu
   std::vector<amj_result_t> released;  // under mu: returned to the pool by the worker
@@ -190,6 +200,8 @@ struct amj_server {
     std::vector<uint32_t> pid;
     for (const Req& q : reqs) {
       o.materialize = q.materialize;
+      const auto known = result_rows.find(shape(q));
+      o.dro_capacity_rows = known != result_rows.end() ? std::max<uint64_t>(known->second, 1) : 0;
       uint32_t id = 0;
       int rc;
       if (q.query == AMJ_QUERY_NORMAL) {
@@ -227,6 +239,7 @@ struct amj_server {
       } else if (r != AMJ_OK) {
         c.out.push_back(failed(planned[i], r, ctx->last_error));
       } else {
+        result_rows[shape(planned[i])] = res.count;
         finished(c, planned[i], res);
       }
     }

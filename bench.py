@@ -239,8 +239,11 @@ def serving_leg(G, n_queries, max_rows, rate, seed=42, budget=200 << 30):
                         f"{max_rows} rows, 20% TRI, 20% FOUR on R-MAT 12..16), Poisson "
                         f"arrivals at {rate} queries/s, results materialised, "
                         f"{budget / 2**30:.0f} GiB HBM budget for held results",
-            "engines": "served: continuous batching, one sample + one direct-output launch per "
-                       "depth per batch; plain: one query per launch, count + scan + write",
+            "engines": "served: continuous batching, one direct-output launch per depth per "
+                       "batch (outputs sized from a sample launch for queries not seen before, "
+                       "from the last exact count for repeated ones); plain: one query per "
+                       "launch, count + scan + write",
+            "distinct_queries": len(set(stream)),
             "verified": "every result (count + checksum) equal in the served and plain runs"}
 
 

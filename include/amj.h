@@ -222,7 +222,8 @@ int amj_batch_info(const amj_batch_t* batch, uint32_t* queries, double* run_ms,
  * registered once (resident, hashed on column 0 with value-sorted runs), requests from any host
  * thread are served with continuous batching (each cycle runs, as one batch, every request that
  * arrived while the previous batch was on the GPU, up to max_batch, as dro batch queries: one
- * sample launch and one direct-output launch per depth), and results are admitted against
+ * direct-output launch per depth, plus one sample launch for queries never served before —
+ * a repeated query is sized from its last exact count), and results are admitted against
  * hbm_budget_bytes (bytes of results held by clients until amj_server_release; 0 = half the
  * device; a result counts with its whole output buffer, which direct output sizes from an
  * estimate). plain = 1 serves one query per launch instead (amj_execute's default engine, the
