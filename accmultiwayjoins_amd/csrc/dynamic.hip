@@ -8,7 +8,7 @@
 // and writes into chunks in a single pass, falling back to residual passes when chunks run out.
 //
 // Here persistent waves run the frontier DFS of probe_engine.h:
-//   * a wave claims units of 64 probe rows from a global counter;
+//   * a wave claims units of 64 probe rows from a global counter, in guided chunks;
 //   * when a level of its frontier has more than `split` children left, it keeps the next
 //     `split` and publishes the rest as tasks of `split` children each; a task is the 64-lane
 //     frontier of that level (bindings, runs, prefix) plus its child range, so the wave that
@@ -40,6 +40,7 @@ struct DynCtl {
   unsigned int flags;            // 1: a wait hit the time limit; 2: output capacity overrun
 };
 
+constexpr uint64_t kMaxClaim = 64;  // units claimed at once by a wave
 constexpr uint32_t kFlagSpin = 1u;
 constexpr uint32_t kFlagOverflow = 2u;
 
@@ -228,15 +229,30 @@ __global__ __launch_bounds__(kBlock) void dyn_kernel(const Plan P, const ProbeOu
   for (;;) {
     // busy counts waves holding claimed work; it only keeps idle waves around to help (a wave
     // that leaves early costs parallelism, never work: a pusher always serves its own tasks)
-    uint64_t u = ~0ull;
-    if (lane_id() == 0 && ld_rlx(&A.ctl->next_unit) < A.units)
-      u = add_relaxed(&A.ctl->next_unit, 1ull);
+    // Units are claimed in guided chunks (a quarter of the remaining units per wave, 1 to 64):
+    // four counter updates per claim instead of per unit, since one cheap unit costs less than
+    // the atomics on these shared counters.
+    uint64_t u = ~0ull, nu = 0;
+    if (lane_id() == 0) {
+      const unsigned long long seen_u = ld_rlx(&A.ctl->next_unit);
+      if (seen_u < A.units) {
+        const uint64_t waves = (uint64_t)gridDim.x * kWavesPerBlock;
+        uint64_t k = (A.units - seen_u) / (4 * waves);
+        k = k < 1 ? 1 : (k > kMaxClaim ? kMaxClaim : k);
+        u = add_relaxed(&A.ctl->next_unit, (unsigned long long)k);
+        nu = k;
+      }
+    }
     u = lane0_u64(u);
+    nu = lane0_u64(nu);
     if (u < A.units) {
+      const uint64_t ue = u + nu < A.units ? u + nu : A.units;
       if (lane_id() == 0) add_relaxed(&A.ctl->busy, 1u);
-      Lv l0;
-      make_level0(P, u * kWave, l0);
-      if (l0.T) dyn_descend<L, 0, MODE>(P, l0, 0, l0.T, out, A, acc);
+      for (uint64_t v = u; v < ue; ++v) {
+        Lv l0;
+        make_level0(P, v * kWave, l0);
+        if (l0.T) dyn_descend<L, 0, MODE>(P, l0, 0, l0.T, out, A, acc);
+      }
       if (lane_id() == 0) {
         add_relaxed(&A.ctl->progress, 1ull);
         add_acq_rel(&A.ctl->busy, ~0u);
