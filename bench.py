@@ -199,6 +199,43 @@ def ooc_leg(G, rows, budget_gb):
     return out
 
 
+def four_leg(G, scale):
+    """4-clique query (FOUR) on an R-MAT graph through AMHJ (amj_execute_pattern) and ALFTJ DFS
+    over one trie aliased six times; count + checksum (at scale 20 the ~10^10 results do not
+    fit one GPU's memory as tuples). The two engines must agree."""
+    from accmultiwayjoins_amd import Context
+    ctx = Context(0)
+    src, dst = G.rmat_oriented(scale, 16, seed=42)
+    rel = ctx.relation([src, dst], [0, 1])
+    ht = ctx.hash_table(rel, 0, sort_vals=True)
+    t = ctx.trie_from_relation(rel)
+    four = [(0, 1), (1, 2), (2, 3), (0, 3), (0, 2), (1, 3)]
+    res = {}
+    for name, fn in (("amhj", lambda: ctx.execute_pattern(rel, ht, "FOUR", materialize=False)),
+                     ("alftj_dfs", lambda: ctx.lftj([t] * 6, [0, 1, 2, 3], "DFS",
+                                                    trie_attrs=four, materialize=False))):
+        t0 = time.perf_counter()
+        r = fn()
+        N = __import__("accmultiwayjoins_amd._native", fromlist=["lib"])
+        N.lib().amj_sync(ctx.h)
+        el = time.perf_counter() - t0
+        res[name] = (r.count, r.checksum, el, r.probe_ms)
+        log(f"[bench] four: {name} {r.count} 4-cliques in {el * 1e3:.0f} ms")
+    (ca, sa, ea, pa), (cl, sl, el_, pl) = res["amhj"], res["alftj_dfs"]
+    if (ca, sa) != (cl, sl):
+        raise SystemExit("bench: FOUR through AMHJ and ALFTJ disagree")
+    t.close()
+    ht.close()
+    rel.close()
+    ctx.close()
+    return {"count": ca, "checksum": "%#018x" % sa, "edges": len(src),
+            "amhj": {"value": ca / (pa / 1e3), "unit": "tuples/s", "probe_ms": round(pa, 3)},
+            "alftj_dfs": {"value": cl / (pl / 1e3), "unit": "tuples/s", "probe_ms": round(pl, 3)},
+            "workload": f"FOUR (4-clique) on R-MAT scale {scale} edge factor 16, {len(src)} "
+                        f"oriented edges, count + checksum (hash table / trie built untimed)",
+            "verified": "AMHJ and ALFTJ DFS agree (count + checksum)"}
+
+
 def serving_leg(G, n_queries, max_rows, rate, seed=42, budget=200 << 30):
     """Mixed stream (60% chains of 3/4/5 relations, 20% TRI, 20% FOUR) over resident relations:
     chain relations log-uniform in [10^4, max_rows] rows with Zipf(s ~ U[0, 1.2]) build-side
@@ -320,6 +357,9 @@ def main():
                     help="also run the out-of-core chain leg with this many rows per relation "
                          "(1500000000 for the 1.5 B-tuple case; off by default: ~1 min)")
     ap.add_argument("--ooc-budget-gb", type=float, default=48.0)
+    ap.add_argument("--four-scale", type=int, default=0,
+                    help="also run FOUR through AMHJ and ALFTJ on R-MAT of this scale (20 for "
+                         "the 4-clique case; off by default)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device three times (torch), to scale the "
                          "FETCH_SIZE / WRITE_SIZE counters of a profiled run, then exit")
@@ -466,6 +506,8 @@ def main():
     ctx.close()
     if args.ooc_rows:
         out["ooc"] = ooc_leg(G, args.ooc_rows, args.ooc_budget_gb)
+    if args.four_scale:
+        out["four"] = four_leg(G, args.four_scale)
     if not args.no_serving:  # its own server context, after the chain's memory is released
         serving = serving_leg(G, args.serve_queries, args.serve_max_rows, args.serve_rate)
         log(f"[bench] serving: {serving['served']['queries_per_s']:.0f} q/s served vs "
