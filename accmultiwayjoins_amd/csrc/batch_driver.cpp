@@ -345,8 +345,13 @@ int run_direct(amj_batch_t* b, const std::vector<uint32_t>& idx,
     }
     for (size_t j = admitted.size(); j < idx.size(); ++j) b->qs[idx[j]].status = AMJ_ERR_NOMEM;
   }
-  // 3. one direct-output pass over the admitted queries
-  const std::vector<uint32_t> order = depth_order(b, admitted);
+  // 3. one direct-output pass over the admitted queries; inside a depth group the largest
+  //    outputs come first, so their long units start early instead of ending the launch alone
+  std::vector<uint32_t> order = admitted;
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+    const int lx = b->qs[x].plan.n_levels, ly = b->qs[y].plan.n_levels;
+    return lx != ly ? lx < ly : capq[x] > capq[y];
+  });
   const uint32_t nq = (uint32_t)order.size();
   std::vector<uint64_t> units(nq), cap(nq), unit0(nq);
   uint64_t total = 0;
