@@ -77,6 +77,57 @@ def cpu_baseline(rels, sample_rows, threads):
     return cnt, (pms + qms) / 1e3, wall
 
 
+def host_cpu():
+    """(model name, logical CPUs, SMT active) of the host."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    smt = None
+    try:
+        with open("/sys/devices/system/cpu/smt/active") as f:
+            smt = f.read().strip() == "1"
+    except OSError:
+        pass
+    return model, os.cpu_count(), smt
+
+
+def copy_bandwidth_gbs():
+    """Device-to-device copy bandwidth (bytes read + written per second; 4 GiB copied, best of
+    4, HIP events). Runs in the torch-only --copy-calibration child process."""
+    import torch
+    src = torch.ones(1 << 30, dtype=torch.int32, device="cuda")
+    dst = torch.empty_like(src)
+    best = None
+    for _ in range(4):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        dst.copy_(src)
+        b.record()
+        b.synchronize()
+        ms = a.elapsed_time(b)
+        best = ms if best is None else min(best, ms)
+    return 2 * 4 * (1 << 30) / (best / 1e3) / 1e9
+
+
+def measured_copy_gbs():
+    """The copy bandwidth measured by a child process (torch and libamj each bring their own
+    HIP runtime, so the measurement does not share a process with the joins); None on failure."""
+    import subprocess
+    try:
+        p = subprocess.run([sys.executable, os.path.abspath(__file__), "--copy-calibration"],
+                           capture_output=True, text=True, timeout=300)
+        return json.loads(p.stdout.strip().splitlines()[-1])["copy_gbs"]
+    except Exception as e:  # noqa: BLE001 - reported as missing
+        log(f"[bench] copy bandwidth not measured: {e!r}")
+        return None
+
+
 def triangle_leg(ctx, G, scale, steps, warmup):
     """Triangle query E(a,b) x E(b,c) x E(a,c) on an R-MAT graph (edge factor 16, parameters
     0.57/0.19/0.19/0.05, symmetrised, deduplicated, degree-oriented): one step = trie build +
@@ -376,6 +427,8 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         log("[bench] copied 3 x 1 GiB")
+        del src, dst
+        print(json.dumps({"copy_gbs": copy_bandwidth_gbs()}), flush=True)
         return
 
     from accmultiwayjoins_amd import Context
@@ -463,17 +516,28 @@ def main():
     if not args.no_triangle:
         tri = triangle_leg(ctx, G, args.tri_scale, args.steps, args.warmup)
 
+    copy_gbs = measured_copy_gbs()
+    log(f"[bench] device-to-device copy: {copy_gbs} GB/s (read + write)")
+
     cpu = None
     if not args.no_cpu:
         s0 = skews[-1]
         sample_rows = min(args.cpu_sample_rows, len(inputs[s0][0][0][0]))
         cnt, secs, wall = cpu_baseline(inputs[s0], sample_rows, args.cpu_threads)
+        one_rows = sample_rows // 8
+        cnt1, secs1, _ = cpu_baseline(inputs[s0], one_rows, 1)
+        model, ncpu, smt = host_cpu()
         cpu = {"value": cnt / secs if secs > 0 else 0.0, "unit": "tuples/s",
                "cores": args.cpu_threads, "kind": "port",
                "sample": f"oracle MHJ-CPU restatement (partition + probe, count+checksum, no "
                          f"materialisation) on the s={s0} inputs with R1 cut to its first "
-                         f"{sample_rows} rows; {cnt} results in {secs:.2f} s"}
-        log(f"[bench] cpu baseline: {cnt} tuples in {secs:.2f} s ({wall:.2f} s wall)")
+                         f"{sample_rows} rows; {cnt} results in {secs:.2f} s",
+               "one_thread": {"value": cnt1 / secs1 if secs1 > 0 else 0.0,
+                              "sample": f"R1 cut to {one_rows} rows; {cnt1} results in "
+                                        f"{secs1:.2f} s"},
+               "host": {"cpu_model": model, "logical_cpus": ncpu, "smt_active": smt}}
+        log(f"[bench] cpu baseline: {cnt} tuples in {secs:.2f} s ({wall:.2f} s wall); "
+            f"1 thread: {cnt1} in {secs1:.2f} s")
 
     out = {
         "metric": "join output tuples/s (4-way chain)",
@@ -494,6 +558,7 @@ def main():
                    "scale": args.scale, "skews": skews, "ws": args.ws, "dro": args.dro,
                    "fib": args.fib, "per_skew": per_skew},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "measured_copy_gbs": copy_gbs,
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT)},
         "cpu_baseline": cpu,
