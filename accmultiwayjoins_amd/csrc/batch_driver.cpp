@@ -279,14 +279,8 @@ int run_exact(amj_batch_t* b, const std::vector<uint32_t>& idx) {
 // Direct output (dro): per-query capacities from a strided sample of units, one pass that
 // reserves output rows with atomics. Materialised queries whose results overran their capacity
 // are returned in *redo (their counts and checksums are exact, their rows incomplete).
-#ifndef AMJ_BATCH_SAMPLE_UNITS
-#define AMJ_BATCH_SAMPLE_UNITS 1024
-#endif
-#ifndef AMJ_BATCH_SLACK_PCT
-#define AMJ_BATCH_SLACK_PCT 125
-#endif
-constexpr uint64_t kSampleUnits = AMJ_BATCH_SAMPLE_UNITS;  // units sampled per query
-constexpr double kSlack = AMJ_BATCH_SLACK_PCT / 100.0;    // estimate -> output size
+constexpr uint64_t kSampleUnits = 1024;  // units sampled per query
+constexpr double kSlack = 1.25;          // estimate -> output size
 
 int run_direct(amj_batch_t* b, const std::vector<uint32_t>& idx,
                std::vector<uint32_t>* redo) {

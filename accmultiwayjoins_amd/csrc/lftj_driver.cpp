@@ -91,10 +91,7 @@ static int make_lplan(amj_ctx_t* ctx, const std::vector<LUse>& uses, const uint3
   return AMJ_OK;
 }
 
-#ifndef AMJ_LFTJ_SAMPLE_UNITS
-#define AMJ_LFTJ_SAMPLE_UNITS 1024
-#endif
-constexpr uint64_t kLftjSampleUnits = AMJ_LFTJ_SAMPLE_UNITS;
+constexpr uint64_t kLftjSampleUnits = 1024;
 
 // DFS with direct output (opts.dro, materialising): the output is sized from the counts of an
 // evenly strided sample of at most kLftjSampleUnits units (x1.5 + 64 Ki rows; exact when every unit is

@@ -47,13 +47,11 @@ hipError_t launch_records_to_suffix(RunRef* rec, uint64_t n, const uint64_t* pw,
 
 // Workgroups of one writer launch; each wave loops over windows when the windows outnumber
 // the grid's waves.
-#ifndef AMJ_WINDOW_MAX_BLOCKS
-#define AMJ_WINDOW_MAX_BLOCKS 65536
-#endif
+constexpr uint64_t kWindowMaxBlocks = 65536;
 uint64_t window_blocks(uint64_t total, uint64_t window) {
   const uint64_t n_windows = (total + window - 1) / window;
   const uint64_t b = n_windows == 0 ? 1 : (n_windows + kWavesPerBlock - 1) / kWavesPerBlock;
-  return b < AMJ_WINDOW_MAX_BLOCKS ? b : AMJ_WINDOW_MAX_BLOCKS;
+  return b < kWindowMaxBlocks ? b : kWindowMaxBlocks;
 }
 
 // The chain layout the writer is specialised for: probe attributes 0..PC-1, table t binds

@@ -48,10 +48,7 @@ typedef __attribute__((address_space(3))) int32_t lds_i32;
 constexpr int kStageRows = 2 * kWave;
 
 // Full 64-result batches the last-level emitter locates before it stores any of them.
-#ifndef AMJ_EMIT_GROUP
-#define AMJ_EMIT_GROUP 8
-#endif
-constexpr int kEmitGroup = AMJ_EMIT_GROUP;
+constexpr int kEmitGroup = 8;
 
 struct AccW {
   uint64_t csum;      // per-lane checksum partial
@@ -72,14 +69,7 @@ __device__ __forceinline__ void flush_block(const ChainOut& out, lds_i32* stage,
   const uint32_t slot = (uint32_t)row & (kStageRows - 1);
   if (lane < n) {
     int32_t* dst = out.base + row;
-#ifdef AMJ_WINDOW_NT_STORE
-    sfor<0, NA>([&](auto a) {
-      __builtin_nontemporal_store((int32_t)stage[a * kStageRows + slot],
-                                  &dst[(uint64_t)a * out.stride]);
-    });
-#else
     sfor<0, NA>([&](auto a) { dst[(uint64_t)a * out.stride] = stage[a * kStageRows + slot]; });
-#endif
   }
   __asm__ __volatile__("" ::: "memory");
 }
@@ -305,11 +295,7 @@ __global__ __launch_bounds__(kBlock) void window_write_kernel(
   using C = Chain<L, PC, NV>;
   constexpr int NA = Chain<L, PC, NV>::NA;
   __shared__ int32_t stage_mem[MAT ? kWavesPerBlock * NA * kStageRows : 1];
-#ifdef AMJ_WINDOW_LINEAR
-  const uint64_t blk = blockIdx.x;
-#else
   const uint64_t blk = xcd_block(blockIdx.x, gridDim.x);  // neighbouring windows per XCD
-#endif
   // windows w, w + n_slots, ...: the grid is capped (a launch holds fewer than 2^32 lanes)
   const uint64_t n_slots = (uint64_t)gridDim.x * kWavesPerBlock;
   AccW acc{0, 0, 0, nullptr, false, false};
