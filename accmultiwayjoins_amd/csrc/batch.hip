@@ -156,7 +156,7 @@ __device__ __forceinline__ void descend_direct(const Plan& P, const Lv& cur, con
   }
 }
 
-template <int L>
+template <int L, bool MAT>
 __global__ __launch_bounds__(kBlock) void batch_direct_kernel(
     const BatchQ* __restrict__ qs, uint32_t nq, uint64_t unit_begin, uint64_t total,
     unsigned long long* __restrict__ rows, unsigned long long* __restrict__ counts,
@@ -166,18 +166,15 @@ __global__ __launch_bounds__(kBlock) void batch_direct_kernel(
   const uint64_t g = unit_begin + w;
   const uint32_t q = uniform(query_of(qs, nq, g));
   const BatchQ& Q = qs[q];
-  __shared__ int32_t stage_mem[kWavesPerBlock * kMaxAttrs * kDirectRows];
-  DirectStage st{(lds_i32*)&stage_mem[(threadIdx.x / kWave) * kMaxAttrs * kDirectRows], 0u};
+  __shared__ int32_t stage_mem[MAT ? kWavesPerBlock * kMaxAttrs * kDirectRows : 1];
+  DirectStage st{(lds_i32*)&stage_mem[MAT ? (threadIdx.x / kWave) * kMaxAttrs * kDirectRows : 0],
+                 0u};
   Lv l0;
   make_level0(Q.P, (g - Q.unit0) * kWave, l0);
   Acc acc{};
-  if (l0.T) {
-    if (Q.out.cols[0] != nullptr) {
-      descend_direct<L, 0, true>(Q.P, l0, Q.out, acc, st, &rows[q]);
-      direct_flush(Q.out, Q.P.n_attrs, st, &rows[q]);
-    } else {
-      descend_direct<L, 0, false>(Q.P, l0, Q.out, acc, st, &rows[q]);
-    }
+  if (l0.T) {  // MAT: every query of this launch materialises (one instantiation per kind)
+    descend_direct<L, 0, MAT>(Q.P, l0, Q.out, acc, st, &rows[q]);
+    if (MAT) direct_flush(Q.out, Q.P.n_attrs, st, &rows[q]);
   }
   const uint64_t cs = wave_sum_u64(acc.csum);
   if (lane_id() == 0) {
@@ -237,17 +234,21 @@ hipError_t launch_batch_write(int L, const void* qs, uint32_t nq, uint64_t unit_
   return hipGetLastError();
 }
 
-hipError_t launch_batch_direct(int L, const void* qs, uint32_t nq, uint64_t unit_begin,
-                               uint64_t total, unsigned long long* rows,
+hipError_t launch_batch_direct(int L, bool mat, const void* qs, uint32_t nq,
+                               uint64_t unit_begin, uint64_t total, unsigned long long* rows,
                                unsigned long long* counts, unsigned long long* csums,
                                hipStream_t s) {
   if (total == 0) return hipSuccess;
   const BatchQ* q = (const BatchQ*)qs;
   switch (L) {
-#define C(LV)                                                                               \
-  case LV:                                                                                  \
-    hipLaunchKernelGGL(batch_direct_kernel<LV>, dim3(grid_units(total)), dim3(kBlock), 0, s, \
-                       q, nq, unit_begin, total, rows, counts, csums);                      \
+#define C(LV)                                                                                  \
+  case LV:                                                                                     \
+    if (mat)                                                                                   \
+      hipLaunchKernelGGL((batch_direct_kernel<LV, true>), dim3(grid_units(total)), dim3(kBlock), \
+                         0, s, q, nq, unit_begin, total, rows, counts, csums);                 \
+    else                                                                                       \
+      hipLaunchKernelGGL((batch_direct_kernel<LV, false>), dim3(grid_units(total)),           \
+                         dim3(kBlock), 0, s, q, nq, unit_begin, total, rows, counts, csums);   \
     break;
     C(1) C(2) C(3) C(4) C(5)
 #undef C

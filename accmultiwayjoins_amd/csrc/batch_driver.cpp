@@ -19,8 +19,8 @@ hipError_t launch_batch_totals(const void* qs, uint32_t nq, const uint64_t* base
 size_t batch_desc_bytes();
 void batch_fill_desc(void* dst, const Plan& P, const ProbeOut& out, uint64_t unit0,
                      uint64_t units, uint64_t stride);
-hipError_t launch_batch_direct(int L, const void* qs, uint32_t nq, uint64_t unit_begin,
-                               uint64_t total, unsigned long long* rows,
+hipError_t launch_batch_direct(int L, bool mat, const void* qs, uint32_t nq,
+                               uint64_t unit_begin, uint64_t total, unsigned long long* rows,
                                unsigned long long* counts, unsigned long long* csums,
                                hipStream_t s);
 
@@ -341,10 +341,12 @@ int run_direct(amj_batch_t* b, const std::vector<uint32_t>& idx,
   }
   // 3. one direct-output pass over the admitted queries; inside a depth group the largest
   //    outputs come first, so their long units start early instead of ending the launch alone
+  //    (one launch per depth and kind: materialising or count-only)
   std::vector<uint32_t> order = admitted;
   std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
     const int lx = b->qs[x].plan.n_levels, ly = b->qs[y].plan.n_levels;
-    return lx != ly ? lx < ly : capq[x] > capq[y];
+    const bool mx = b->qs[x].materialize, my = b->qs[y].materialize;
+    return lx != ly ? lx < ly : (mx != my ? mx : capq[x] > capq[y]);
   });
   const uint32_t nq = (uint32_t)order.size();
   std::vector<uint64_t> units(nq), cap(nq), unit0(nq);
@@ -370,12 +372,16 @@ int run_direct(amj_batch_t* b, const std::vector<uint32_t>& idx,
   unsigned long long* csums = rows + 2 * nq;
   AMJ_HIP_TRY(ctx, hipMemcpyAsync(ddesc, desc.data(), desc.size(), hipMemcpyHostToDevice, s));
   AMJ_HIP_TRY(ctx, hipMemsetAsync(acc, 0, (size_t)nq * 24, s));
-  AMJ_TRY(for_depth_groups(b, order, [&](uint32_t k, uint32_t e, int L) -> int {
+  for (uint32_t k = 0; k < nq;) {
+    uint32_t e = k;
+    const int L = b->qs[order[k]].plan.n_levels;
+    const bool mat = b->qs[order[k]].materialize;
+    while (e < nq && b->qs[order[e]].plan.n_levels == L && b->qs[order[e]].materialize == mat) ++e;
     const uint64_t gu = unit0[e - 1] + units[e - 1] - unit0[k];
-    AMJ_HIP_TRY(ctx, launch_batch_direct(L, (const uint8_t*)ddesc + db * k, e - k, unit0[k], gu,
-                                         rows + k, counts + k, csums + k, s));
-    return AMJ_OK;
-  }));
+    AMJ_HIP_TRY(ctx, launch_batch_direct(L, mat, (const uint8_t*)ddesc + db * k, e - k, unit0[k],
+                                         gu, rows + k, counts + k, csums + k, s));
+    k = e;
+  }
   std::vector<uint64_t> h(3 * (size_t)nq);
   AMJ_HIP_TRY(ctx, hipMemcpyAsync(h.data(), acc, (size_t)nq * 24, hipMemcpyDeviceToHost, s));
   AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
