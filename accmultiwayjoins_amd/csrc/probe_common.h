@@ -94,9 +94,16 @@ __device__ __forceinline__ void lookup_run(const DevTable& tb, int32_t key, bool
     if (last != k) e = upper_bound_u32(tb.keys, s, e - 1, k);
   }
   if (check && s < e) {
+    // the run of cval inside the value-sorted key run; a second search only when cval occurs
+    // more than once (duplicate pairs), so a membership test costs one search plus one load
     const uint32_t s2 = lower_bound_i32(tb.vals, s, e, cval);
-    e = upper_bound_i32(tb.vals, s2, e, cval);
+    uint32_t e2 = s2;
+    if (s2 < e && tb.vals[s2] == cval) {
+      e2 = s2 + 1;
+      if (e2 < e && tb.vals[e2] == cval) e2 = upper_bound_i32(tb.vals, e2, e, cval);
+    }
     s = s2;
+    e = e2;
   }
   lo = s;
   len = e - s;
