@@ -177,6 +177,18 @@ GEN_SIGNATURES = [
      [C.c_void_p, C.c_uint64, C.c_int64, C.c_int64, C.c_uint64, C.c_uint64]),
     ("amj_gen_zipf", C.c_int,
      [C.c_void_p, C.c_uint64, C.c_uint64, C.c_double, C.c_uint64, C.c_uint64]),
+    ("amj_gen_zipf_mapped", C.c_int,
+     [C.c_void_p, C.c_uint64, C.c_uint64, C.c_double, C.c_uint64, C.c_uint64, C.c_int]),
+    ("amj_gen_unique_column", C.c_int,
+     [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64]),
+    ("amj_gen_unique_pairs", C.c_int,
+     [C.c_void_p, C.c_void_p, C.c_uint64, C.c_int64, C.c_int64, C.c_int64, C.c_int64,
+      C.c_uint64, C.c_int]),
+    ("amj_gen_zipf_pairs", C.c_int,
+     [C.c_void_p, C.c_void_p, C.c_uint64, C.c_int64, C.c_int64, C.c_int64, C.c_int64,
+      C.c_uint64, C.c_int, C.c_double, C.c_int]),
+    ("amj_gen_seq_pairs", C.c_int,
+     [C.c_void_p, C.c_void_p, C.c_uint64, C.c_int64, C.c_int64]),
     ("amj_gen_rmat", None,
      [C.c_void_p, C.c_void_p, C.c_int, C.c_uint64, C.c_double, C.c_double, C.c_double,
       C.c_uint64]),

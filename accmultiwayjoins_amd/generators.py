@@ -3,7 +3,10 @@
 Workloads (sizes as used by the tests and bench):
   * chain joins R1(a,b) x R2(b,c) x ... with int32 columns; join columns uniform or Zipf(s)
     over K keys with a seeded rank->key permutation;
-  * R-MAT graphs (Graph500 style relabelled), symmetrised, deduplicated and degree-oriented.
+  * R-MAT graphs (Graph500 style relabelled), symmetrised, deduplicated and degree-oriented;
+  * the reference's generator shapes: columns of unique-modulo values, distinct uniform pairs,
+    distinct pairs with one Zipf column, sequential pairs (the writer command lines of
+    writers.py build .db files from them).
 """
 from __future__ import annotations
 
@@ -27,6 +30,58 @@ def zipf(n: int, k: int, s: float, seed: int, stream: int) -> np.ndarray:
     if gen().amj_gen_zipf(_ptr(out), n, k, float(s), seed, stream) != 0:
         raise ValueError("invalid Zipf key count")
     return out
+
+
+def zipf_mapped(n: int, k: int, s: float, seed: int, stream: int, shuffle: bool) -> np.ndarray:
+    """Zipf(s) over k values; shuffle=False maps rank r to value r-1 (value 0 most frequent)."""
+    out = np.empty(n, dtype=np.int32)
+    if gen().amj_gen_zipf_mapped(_ptr(out), n, k, float(s), seed, stream, int(shuffle)) != 0:
+        raise ValueError("invalid Zipf key count")
+    return out
+
+
+def unique_column(n: int, value_range: int, seed: int, stream: int) -> np.ndarray:
+    """j % value_range for j in [0, n), in a seeded random order."""
+    out = np.empty(n, dtype=np.int32)
+    if gen().amj_gen_unique_column(_ptr(out), n, value_range, seed, stream) != 0:
+        raise ValueError("invalid range or size for a unique column")
+    return out
+
+
+def _pair_rc(rc: int, what: str):
+    if rc == -1:
+        raise ValueError(f"{what}: invalid ranges, or not fewer rows than distinct pairs")
+    if rc != 0:
+        raise ValueError(f"{what}: the ranges cannot hold that many distinct pairs")
+
+
+def unique_pairs(n: int, range0, range1, seed: int = 42, shuffle: bool = False):
+    """n distinct pairs uniform over range0 x range1 ((lo, hi) tuples), sorted unless shuffled."""
+    k0 = np.empty(n, dtype=np.int32)
+    k1 = np.empty(n, dtype=np.int32)
+    _pair_rc(gen().amj_gen_unique_pairs(_ptr(k0), _ptr(k1), n, range0[0], range0[1], range1[0],
+                                        range1[1], seed, int(shuffle)), "unique pairs")
+    return k0, k1
+
+
+def zipf_pairs(n: int, range0, range1, skew_col: int, z: float, seed: int = 42,
+               shuffle: bool = False):
+    """n distinct pairs: column skew_col Zipf(z), the other uniform and distinct per value."""
+    k0 = np.empty(n, dtype=np.int32)
+    k1 = np.empty(n, dtype=np.int32)
+    _pair_rc(gen().amj_gen_zipf_pairs(_ptr(k0), _ptr(k1), n, range0[0], range0[1], range1[0],
+                                      range1[1], seed, skew_col, float(z), int(shuffle)),
+             "Zipf pairs")
+    return k0, k1
+
+
+def seq_pairs(n: int, range0):
+    """(lo + i // p, i % p) with p = n // (hi - lo)."""
+    k0 = np.empty(n, dtype=np.int32)
+    k1 = np.empty(n, dtype=np.int32)
+    if gen().amj_gen_seq_pairs(_ptr(k0), _ptr(k1), n, range0[0], range0[1]) != 0:
+        raise ValueError("sequential pairs need at least one row per key0 value")
+    return k0, k1
 
 
 def chain_relations(sizes, key_range, seed=42, zipf_s=None, payload_range=None):
