@@ -20,7 +20,7 @@ STATUS = {
     -4: "AMJ_ERR_NODEV", -5: "AMJ_ERR_STATE",
 }
 PATHS = {1: "plain", 2: "fib", 3: "ws", 4: "dro", 5: "lftj-dfs", 6: "lftj-bfs", 7: "ws-dro", 8: "batch",
-         9: "ooc", 10: "lftj-ooc"}
+         9: "ooc", 10: "lftj-ooc", 11: "tpt"}
 
 QUERY = {"NORMAL": 0, "TRI": 1, "FOUR": 2}
 AMJ_REL_HOST = 2
@@ -52,7 +52,8 @@ class AmjOpts(C.Structure):
         ("ws_split_threshold", C.c_uint32),
         ("fib_suffix", C.c_int),
         ("probe_order", C.c_int),
-        ("reserved", C.c_uint32 * 5),
+        ("engine", C.c_uint32),
+        ("reserved", C.c_uint32 * 4),
     ]
 
 

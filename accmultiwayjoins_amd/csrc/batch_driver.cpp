@@ -70,6 +70,10 @@ static int batch_add_plan(amj_batch_t* b, Plan& plan, std::vector<uint32_t>& ids
     set_error(b->ctx, AMJ_ERR_STATE, "the batch has already run");
     return AMJ_ERR_STATE;
   }
+  if (opts && opts->engine != AMJ_ENGINE_WAVE) {
+    set_error(b->ctx, AMJ_ERR_ARG, "batches run the wave engine only (opts.engine = 0)");
+    return AMJ_ERR_ARG;
+  }
   amj_batch::Query q;
   q.plan = plan;
   q.attr_ids = ids;

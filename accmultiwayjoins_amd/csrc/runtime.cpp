@@ -466,6 +466,64 @@ static int run_plain(amj_ctx_t* ctx, Plan& P, const amj_opts_t* o, amj_result_t*
   return AMJ_OK;
 }
 
+// Thread-per-tuple baseline: count pass, scan of the per-lane counts, write pass (the
+// count pass also hashes when nothing is materialised).
+static int run_tpt(amj_ctx_t* ctx, Plan& P, const amj_opts_t* o, amj_result_t* res) {
+  hipStream_t s = ctx->stream;
+  const uint64_t nb = tpt_blocks(P), lanes = nb * kBlock;
+  void *counts = nullptr, *tmp = nullptr, *bsum = nullptr;
+  AMJ_TRY(scratch(ctx, SLOT_UNIT_COUNTS, (lanes + 1) * 8, &counts));
+  AMJ_TRY(scratch(ctx, SLOT_SCAN_TEMP, scan_temp_elems(lanes + 1) * 8, &tmp));
+  AMJ_TRY(scratch(ctx, SLOT_BLOCK_SUMS, nb * 8, &bsum));
+  const bool mat = o->materialize != 0;
+  const bool hash_in_count = !mat;
+  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[2], s));
+  AMJ_HIP_TRY(ctx, hipMemsetAsync((uint64_t*)counts + lanes, 0, 8, s));
+  ProbeOut none{};
+  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
+  AMJ_HIP_TRY(ctx, launch_tpt(P, false, hash_in_count, (uint64_t*)counts, nullptr, none,
+                              (uint64_t*)bsum, s));
+  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
+  AMJ_HIP_TRY(ctx, exclusive_scan_u64((uint64_t*)counts, (uint64_t*)counts, lanes + 1,
+                                      (uint64_t*)tmp, s));
+  uint64_t total = 0;
+  AMJ_HIP_TRY(ctx, hipMemcpyAsync(&total, (uint64_t*)counts + lanes, 8, hipMemcpyDeviceToHost, s));
+  AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
+  res->count = total;
+  double kernel_ms = elapsed_ms(ctx->ev[3], ctx->ev[4]);
+  ProbeOut out{};
+  if (mat && total > 0) {
+    const uint64_t need = total * (uint64_t)P.n_attrs * 4;
+    if (need > budget_left(ctx, o)) {
+      set_error(ctx, AMJ_ERR_NOMEM, "result of " + std::to_string(total) +
+                                        " tuples does not fit the HBM budget (use materialize=0)");
+      return AMJ_ERR_NOMEM;
+    }
+    AMJ_TRY(alloc_output(ctx, total, P.n_attrs, res));
+    for (int a = 0; a < P.n_attrs; ++a) out.cols[a] = res->cols[a];
+    out.capacity = total;
+    const bool hash = o->checksum != 0;
+    AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
+    AMJ_HIP_TRY(ctx, launch_tpt(P, true, hash, nullptr, (const uint64_t*)counts, out,
+                                (uint64_t*)bsum, s));
+    AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
+    if (hash) AMJ_TRY(finish_checksum(ctx, (const uint64_t*)bsum, nb, &res->checksum));
+    AMJ_HIP_TRY(ctx, hipEventSynchronize(ctx->ev[4]));
+    kernel_ms += elapsed_ms(ctx->ev[3], ctx->ev[4]);
+  } else if (hash_in_count) {
+    AMJ_TRY(finish_checksum(ctx, (const uint64_t*)bsum, nb, &res->checksum));
+  }  // else: materialising an empty result, checksum 0
+  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[5], s));
+  AMJ_HIP_TRY(ctx, hipEventSynchronize(ctx->ev[5]));
+  res->probe_ms = elapsed_ms(ctx->ev[2], ctx->ev[5]);
+  res->probe_kernel_ms = kernel_ms;
+  res->probe_kernel_launches = mat && total > 0 ? 2 : 1;
+  res->rows_written = out.cols[0] ? total : 0;
+  res->path = AMJ_PATH_TPT;
+  res->passes = mat && total > 0 ? 2 : 1;
+  return AMJ_OK;
+}
+
 int execute_plan(amj_ctx_t* ctx, Plan& plan, const std::vector<uint32_t>& attr_ids,
                  const amj_opts_t* opts, amj_result_t* res) {
   amj_opts_t def;
@@ -474,6 +532,18 @@ int execute_plan(amj_ctx_t* ctx, Plan& plan, const std::vector<uint32_t>& attr_i
   std::memset(res, 0, sizeof(*res));
   res->num_cols = (uint32_t)plan.n_attrs;
   for (int a = 0; a < plan.n_attrs; ++a) res->attr_ids[a] = attr_ids[a];
+  if (o->engine == AMJ_ENGINE_TPT) {
+    if (o->ooc || plan.probe_on_host || o->ws || o->dro || o->fib) {
+      set_error(ctx, AMJ_ERR_ARG,
+                "engine AMJ_ENGINE_TPT runs alone: clear ooc, ws, dro and fib");
+      return AMJ_ERR_ARG;
+    }
+    return run_tpt(ctx, plan, o, res);
+  }
+  if (o->engine != AMJ_ENGINE_WAVE) {
+    set_error(ctx, AMJ_ERR_ARG, "unknown engine " + std::to_string(o->engine));
+    return AMJ_ERR_ARG;
+  }
   if (o->ooc || plan.probe_on_host) return run_ooc(ctx, plan, o, res);
   if (o->fib && fib_eligible(plan) && window_plan_supported(plan)) {
     const int rc = run_fib(ctx, plan, o, res);

@@ -203,6 +203,26 @@ def triangle_leg(ctx, G, scale, steps, warmup):
                          "verified_against": "ALFTJ DFS (count + checksum)"}}
 
 
+def ablation_leg(ctx, resident, s, checked, wave_probe_ms):
+    """The thread-per-tuple baseline engine (one lane per probe row, no cooperation in the
+    wave) on the same resident chain at skew s, materialising, against the wave engine's
+    probe time from the timed steps; its count and checksum must equal the verified ones."""
+    R = resident[s]
+    H = [ctx.hash_table(r) for r in R[1:]]
+    r = ctx.execute(R[0], H, engine="tpt", materialize=True, checksum=True)
+    ctx.free(r)
+    for h in H:
+        h.close()
+    if (r.count, r.checksum) != checked[:2]:
+        raise SystemExit(f"bench: thread-per-tuple result differs at s={s}")
+    log(f"[bench] ablation s={s}: thread-per-tuple probe {r.probe_ms:.1f} ms vs wave engine "
+        f"{wave_probe_ms:.1f} ms")
+    return {"skew": s, "thread_per_tuple_probe_ms": round(r.probe_ms, 3),
+            "wave_probe_ms": wave_probe_ms,
+            "speedup": round(r.probe_ms / wave_probe_ms, 2) if wave_probe_ms > 0 else None,
+            "verified": True}
+
+
 def ooc_leg(G, rows, budget_gb):
     """Out-of-core 3-way chain R1(a,b) x R2(b,c) x R3(c,d) with `rows` tuples per relation and
     join keys uniform over [0, rows): the probe relation stays in pinned host memory and is
@@ -400,6 +420,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-triangle", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-ablation", action="store_true",
+                    help="skip the thread-per-tuple baseline run on the first skew")
     ap.add_argument("--serve-queries", type=int, default=512)
     ap.add_argument("--serve-max-rows", type=int, default=2_000_000)
     ap.add_argument("--serve-rate", type=float, default=4000.0)
@@ -512,6 +534,11 @@ def main():
                          "verified_against": checked[s][3]}
                 for (s, cnt, cs, bms, pms, kms, path, _x) in last}
 
+    ablation = None
+    if not args.no_ablation:
+        ablation = ablation_leg(ctx, resident, skews[0], checked[skews[0]],
+                                per_skew[str(skews[0])]["probe_ms"])
+
     tri = None
     if not args.no_triangle:
         tri = triangle_leg(ctx, G, args.tri_scale, args.steps, args.warmup)
@@ -562,6 +589,7 @@ def main():
                      "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT)},
         "cpu_baseline": cpu,
+        "ablation": ablation,
         "triangle": tri,
         "serving": None,
     }

@@ -88,8 +88,13 @@ typedef struct {
   int probe_order;              /* fib: 0 group probe rows by join key when the probe has at
                                    least 2^20 rows, 1 keep the input order (the unit engine's
                                    output order), 2 always group */
-  uint32_t reserved[5];
+  uint32_t engine;              /* AMJ_ENGINE_WAVE (0): the wave engines above;
+                                   AMJ_ENGINE_TPT: thread-per-tuple baseline (MHJ_count /
+                                   MHJ_write, MHJ-GPU/Joins/MHJ.cuh:17-321), for comparison */
+  uint32_t reserved[4];
 } amj_opts_t;
+#define AMJ_ENGINE_WAVE 0
+#define AMJ_ENGINE_TPT 1
 
 /* Result of one join. Columns are device memory owned by the library until
  * amj_result_free(); column a holds attribute a (renumbered in first-occurrence order). */
@@ -120,6 +125,7 @@ typedef struct {
 #define AMJ_PATH_BATCH 8        /* many queries per launch (amj_batch_run) */
 #define AMJ_PATH_OOC 9          /* probe relation streamed in slices (opts.ooc / AMJ_REL_HOST) */
 #define AMJ_PATH_LFTJ_OOC 10    /* triejoin per combination of level-0 value ranges */
+#define AMJ_PATH_TPT 11         /* thread-per-tuple baseline (opts.engine = AMJ_ENGINE_TPT) */
 
 /* ---- context ------------------------------------------------------------------------ */
 int amj_ctx_create(int device, amj_ctx_t** out);
@@ -200,7 +206,7 @@ int amj_batch_create(amj_ctx_t* ctx, amj_batch_t** out);
 void amj_batch_destroy(amj_batch_t* batch);
 /* Queue a NORMAL query (the arguments of amj_execute); *id numbers it within the batch.
  * opts->materialize, opts->dro and opts->dro_capacity_rows are used (the batch engine is the
- * unit engine; ws, fib and ooc do not apply). */
+ * unit engine; ws, fib and ooc do not apply; opts->engine must be AMJ_ENGINE_WAVE). */
 int amj_batch_add(amj_batch_t* batch, const amj_relation_t* probe, amj_hash_table_t* const* hts,
                   uint32_t n_hts, const amj_opts_t* opts, uint32_t* id);
 /* Queue a TRI / FOUR query over an edge relation and its table (key column 0, sort_vals=1). */

@@ -18,10 +18,11 @@ GOLDEN = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "mhj_c
 PATHS = [dict(), dict(fib=1), dict(fib=1, fib_suffix=1), dict(fib=1, fib_suffix=2),
          dict(fib=1, probe_order=2),
          dict(ws=1), dict(ws=1, ws_split_threshold=64), dict(dro=1),
-         dict(dro=1, dro_capacity_rows=64), dict(ws=1, dro=1, ws_split_threshold=128)]
+         dict(dro=1, dro_capacity_rows=64), dict(ws=1, dro=1, ws_split_threshold=128),
+         dict(engine="tpt")]
 PATH_IDS = ["plain", "fib", "fib-nosuffix", "fib-deepsuffix", "fib-grouped", "ws", "ws-split64",
             "dro",
-            "dro-overflow", "ws-dro"]
+            "dro-overflow", "ws-dro", "thread-per-tuple"]
 
 
 def _rels(rec):
@@ -154,6 +155,25 @@ def test_pattern_tri_four_match_graph_oracle(ctx, scale):
     r3m = ctx.pattern_join(s, d, "TRI", fetch=True)
     assert r3m.count == n3
     assert O.checksum(r3m.tuples()) == c3
+
+
+def test_thread_per_tuple_baseline_patterns_and_rules(ctx):
+    # the baseline engine runs TRI / FOUR plans (compared columns) to the same results, and
+    # refuses to be combined with the wave engines' switches
+    s, d = G.rmat_oriented(10, 16, seed=42)
+    n3, c3 = O.triangles(s, d)
+    n4, c4 = O.four_cliques(s, d)
+    r3 = ctx.pattern_join(s, d, "TRI", fetch=True, engine="tpt")
+    assert r3.path == "tpt" and r3.count == n3 and r3.checksum == c3
+    assert O.checksum(r3.tuples()) == c3
+    r4 = ctx.pattern_join(s, d, "FOUR", materialize=False, engine="tpt")
+    assert (r4.count, r4.checksum) == (n4, c4) and r4.passes == 1
+    rels = G.chain_relations([5000, 3000, 3000], 500, seed=9)
+    with pytest.raises(AmjError, match="runs alone"):
+        ctx.chain_join(rels, engine="tpt", ws=1)
+    empty = [(c if i != 1 else [x[:0] for x in c], ids) for i, (c, ids) in enumerate(rels)]
+    r = ctx.chain_join(empty, engine="tpt")
+    assert (r.count, r.checksum, r.rows_written) == (0, 0, 0)
 
 
 def test_resident_relations_and_tables_reused(ctx):
