@@ -7,7 +7,7 @@ serves submitted requests with continuous batching under an HBM budget for the r
 by clients; `plain=True` serves one query per launch instead (the comparison path).
 """
 import ctypes as C
-from typing import List, Optional, Sequence, Tuple
+from typing import Sequence
 
 import numpy as np
 
