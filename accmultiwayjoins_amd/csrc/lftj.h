@@ -52,13 +52,19 @@ int trie_from_host(amj_ctx_t* ctx, uint64_t n0, const int32_t* d0, const uint64_
 void free_trie(amj_ctx_t* ctx, TrieDev* t);
 
 // DFS engine (lftj.hip): per-unit counts of 64 roots, then the write pass at scanned bases.
+// `order` (nullptr: units in number order) is a launch order from launch_lftj_unit_weights
+// sorted ascending: wave slot k runs the unit in the low 32 bits of order[k].
 uint64_t lftj_units(const LPlan& P);
-hipError_t launch_lftj_count(const LPlan& P, uint64_t* unit_counts, hipStream_t s);
+// keys[unit] = (~weight << 32) | unit for every unit (see lftj_unit_weight_kernel).
+hipError_t launch_lftj_unit_weights(const LPlan& P, uint64_t* keys, hipStream_t s);
+hipError_t launch_lftj_count(const LPlan& P, uint64_t* unit_counts, const uint64_t* order,
+                             hipStream_t s);
 struct LOut {
   int32_t* c[kMaxLftjAttrs];  // result columns (c[0] == nullptr: checksum only)
 };
 hipError_t launch_lftj_write(const LPlan& P, const uint64_t* unit_base, const LOut& cols,
-                             bool hash, uint64_t* block_sums, uint64_t* n_blocks, hipStream_t s);
+                             bool hash, uint64_t* block_sums, uint64_t* n_blocks,
+                             const uint64_t* order, hipStream_t s);
 // Counts of `units` units k * stride + offset (an evenly strided sample when stride > 1).
 hipError_t launch_lftj_count(const LPlan& P, uint64_t* counts, hipStream_t s, uint64_t units,
                              uint64_t stride, uint64_t offset);
@@ -66,7 +72,7 @@ hipError_t launch_lftj_count(const LPlan& P, uint64_t* counts, hipStream_t s, ui
 // or past `capacity` are dropped, so *rows > capacity means the pass must run again.
 hipError_t launch_lftj_direct(const LPlan& P, const LOut& cols, uint64_t capacity, bool hash,
                               unsigned long long* rows, uint64_t* block_sums, uint64_t* n_blocks,
-                              hipStream_t s);
+                              const uint64_t* order, hipStream_t s);
 
 // Level-0 arrays of a relation readable from the host.
 struct HostView {

@@ -155,19 +155,45 @@ __device__ __forceinline__ void descend(const LPlan& P, const LLv<A>& par, Sink&
   }
 }
 
-// Count pass over units k * stride + offset (stride > 1: an evenly strided sample).
+
+// Unit weights for the launch order: the squared candidate count at attribute 1, summed over
+// the unit's roots (what a root's second level costs when every candidate opens a list of its
+// own), as the key (~min(weight, 2^32-1) << 32) | unit, so an ascending sort puts the heaviest
+// units first and the dispatcher starts them before the light ones.
+template <int A>
+__global__ __launch_bounds__(kBlock) void lftj_unit_weight_kernel(const LPlan P,
+                                                                  uint64_t* __restrict__ keys,
+                                                                  uint64_t units) {
+  const uint64_t k = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+  if (k >= units) return;  // wave-uniform; no barriers in this kernel
+  LLv<A> l0;
+  level0(P, k, l0);
+  const uint64_t w = wave_sum_u64((uint64_t)l0.dlen * (uint64_t)l0.dlen);
+  const uint64_t c = w > 0xffffffffull ? 0xffffffffull : w;
+  if (lane_id() == 0) keys[k] = ((0xffffffffull - c) << 32) | k;
+}
+
+// Unit processed by wave slot k: order[k] holds it in its low 32 bits (nullptr: unit k).
+__device__ __forceinline__ uint64_t unit_of(const uint64_t* order, uint64_t k) {
+  return order ? (uint64_t)(uint32_t)order[k] : k;
+}
+
+// Count pass over units k * stride + offset (stride > 1: an evenly strided sample), counts[k];
+// or, with an order, over every unit in that order, counts[unit].
 template <int A>
 __global__ __launch_bounds__(kBlock) void lftj_count_kernel(const LPlan P,
                                                             uint64_t* __restrict__ counts,
                                                             uint64_t units, uint64_t stride,
-                                                            uint64_t offset) {
+                                                            uint64_t offset,
+                                                            const uint64_t* __restrict__ order) {
   const uint64_t k = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
   if (k >= units) return;  // no barriers in this kernel
+  const uint64_t unit = order ? unit_of(order, k) : k * stride + offset;
   LLv<A> l0;
-  level0(P, k * stride + offset, l0);
+  level0(P, unit, l0);
   CountSink sink;
   if (l0.T) descend<A, 1>(P, l0, sink);
-  if (lane_id() == 0) counts[k] = sink.count;
+  if (lane_id() == 0) counts[order ? unit : k] = sink.count;
 }
 
 template <int A, int MODE>
@@ -175,10 +201,12 @@ __global__ __launch_bounds__(kBlock) void lftj_write_kernel(const LPlan P,
                                                             const uint64_t* __restrict__ base,
                                                             const LOut cols,
                                                             uint64_t* __restrict__ block_sums,
-                                                            uint64_t units) {
-  const uint64_t unit = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+                                                            uint64_t units,
+                                                            const uint64_t* __restrict__ order) {
+  const uint64_t k = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
   WriteSink<MODE> sink{cols, 0, 0};
-  if (unit < units) {
+  if (k < units) {
+    const uint64_t unit = unit_of(order, k);
     LLv<A> l0;
     level0(P, unit, l0);
     sink.out_base = uniform(base[unit]);
@@ -192,15 +220,16 @@ __global__ __launch_bounds__(kBlock) void lftj_direct_kernel(const LPlan P, cons
                                                              uint64_t capacity,
                                                              unsigned long long* __restrict__ rows,
                                                              uint64_t* __restrict__ block_sums,
-                                                             uint64_t units) {
+                                                             uint64_t units,
+                                                             const uint64_t* __restrict__ order) {
   __shared__ int32_t stage_mem[kWavesPerBlock * A * kLftjDirectRows];
-  const uint64_t unit = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+  const uint64_t k = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
   DirectSink<HASH> sink{cols, capacity, rows,
                         (lds_i32*)&stage_mem[(threadIdx.x / kWave) * A * kLftjDirectRows], 0u,
                         A, 0};
-  if (unit < units) {
+  if (k < units) {
     LLv<A> l0;
-    level0(P, unit, l0);
+    level0(P, unit_of(order, k), l0);
     if (l0.T) descend<A, 1>(P, l0, sink);
     sink.flush();
   }
@@ -209,33 +238,56 @@ __global__ __launch_bounds__(kBlock) void lftj_direct_kernel(const LPlan P, cons
 
 uint64_t lftj_units(const LPlan& P) { return (P.roots + kWave - 1) / kWave; }
 
+static hipError_t launch_count(const LPlan& P, uint64_t* counts, hipStream_t s, uint64_t units,
+                               uint64_t stride, uint64_t offset, const uint64_t* order) {
+  if (units == 0) return hipSuccess;
+  const dim3 grid((unsigned)((units + kWavesPerBlock - 1) / kWavesPerBlock));
+#define LC(AV)                                                                                \
+  hipLaunchKernelGGL(lftj_count_kernel<AV>, grid, dim3(kBlock), 0, s, P, counts, units, stride, \
+                     offset, order)
+  switch (P.n_attrs) {
+    case 2: LC(2); break;
+    case 3: LC(3); break;
+    case 4: LC(4); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef LC
+  return hipGetLastError();
+}
+
 hipError_t launch_lftj_count(const LPlan& P, uint64_t* counts, hipStream_t s, uint64_t units,
                              uint64_t stride, uint64_t offset) {
+  return launch_count(P, counts, s, units, stride, offset, nullptr);
+}
+
+hipError_t launch_lftj_count(const LPlan& P, uint64_t* counts, const uint64_t* order,
+                             hipStream_t s) {
+  return launch_count(P, counts, s, lftj_units(P), 1, 0, order);
+}
+
+hipError_t launch_lftj_unit_weights(const LPlan& P, uint64_t* keys, hipStream_t s) {
+  const uint64_t units = lftj_units(P);
   if (units == 0) return hipSuccess;
   const dim3 grid((unsigned)((units + kWavesPerBlock - 1) / kWavesPerBlock));
   switch (P.n_attrs) {
-    case 2: hipLaunchKernelGGL(lftj_count_kernel<2>, grid, dim3(kBlock), 0, s, P, counts, units, stride, offset); break;
-    case 3: hipLaunchKernelGGL(lftj_count_kernel<3>, grid, dim3(kBlock), 0, s, P, counts, units, stride, offset); break;
-    case 4: hipLaunchKernelGGL(lftj_count_kernel<4>, grid, dim3(kBlock), 0, s, P, counts, units, stride, offset); break;
+    case 2: hipLaunchKernelGGL(lftj_unit_weight_kernel<2>, grid, dim3(kBlock), 0, s, P, keys, units); break;
+    case 3: hipLaunchKernelGGL(lftj_unit_weight_kernel<3>, grid, dim3(kBlock), 0, s, P, keys, units); break;
+    case 4: hipLaunchKernelGGL(lftj_unit_weight_kernel<4>, grid, dim3(kBlock), 0, s, P, keys, units); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-hipError_t launch_lftj_count(const LPlan& P, uint64_t* counts, hipStream_t s) {
-  return launch_lftj_count(P, counts, s, lftj_units(P), 1, 0);
-}
-
 hipError_t launch_lftj_direct(const LPlan& P, const LOut& cols, uint64_t capacity, bool hash,
                               unsigned long long* rows, uint64_t* block_sums,
-                              uint64_t* n_blocks, hipStream_t s) {
+                              uint64_t* n_blocks, const uint64_t* order, hipStream_t s) {
   const uint64_t units = lftj_units(P);
   const uint64_t blocks = units == 0 ? 1 : (units + kWavesPerBlock - 1) / kWavesPerBlock;
   *n_blocks = blocks;
   const dim3 grid((unsigned)blocks);
 #define LD(AV, H)                                                                             \
   hipLaunchKernelGGL((lftj_direct_kernel<AV, H>), grid, dim3(kBlock), 0, s, P, cols, capacity, \
-                     rows, block_sums, units)
+                     rows, block_sums, units, order)
   switch (P.n_attrs) {
     case 2: if (hash) LD(2, true); else LD(2, false); break;
     case 3: if (hash) LD(3, true); else LD(3, false); break;
@@ -247,14 +299,16 @@ hipError_t launch_lftj_direct(const LPlan& P, const LOut& cols, uint64_t capacit
 }
 
 hipError_t launch_lftj_write(const LPlan& P, const uint64_t* base, const LOut& cols,
-                             bool hash, uint64_t* block_sums, uint64_t* n_blocks, hipStream_t s) {
+                             bool hash, uint64_t* block_sums, uint64_t* n_blocks,
+                             const uint64_t* order, hipStream_t s) {
   const uint64_t units = lftj_units(P);
   const uint64_t blocks = units == 0 ? 1 : (units + kWavesPerBlock - 1) / kWavesPerBlock;
   *n_blocks = blocks;
   const dim3 grid((unsigned)blocks);
   const int mode = cols.c[0] ? (hash ? MODE_WRITE_HASH : MODE_WRITE) : MODE_HASH;
 #define LW(AV, M) \
-  hipLaunchKernelGGL((lftj_write_kernel<AV, M>), grid, dim3(kBlock), 0, s, P, base, cols, block_sums, units)
+  hipLaunchKernelGGL((lftj_write_kernel<AV, M>), grid, dim3(kBlock), 0, s, P, base, cols, block_sums, \
+                     units, order)
 #define LWA(AV)                                              \
   if (mode == MODE_WRITE) LW(AV, MODE_WRITE);                \
   else if (mode == MODE_WRITE_HASH) LW(AV, MODE_WRITE_HASH); \

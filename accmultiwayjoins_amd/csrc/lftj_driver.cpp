@@ -98,6 +98,27 @@ constexpr uint64_t kLftjSampleUnits = 1024;
 // sampled, or opts.dro_capacity_rows), one pass writes through per-wave LDS buffers with one
 // atomic reservation per 256 rows, and a pass that overran its output runs once more into an
 // exact one. Count and checksum are those of the count/write engine; the row order is not.
+// Launch order of the DFS units, heaviest first, when there are more units than wave slots
+// (otherwise every unit starts at once and the order cannot shorten the tail): a unit whose
+// roots open long candidate lists otherwise starts late and runs alone at the end.
+constexpr uint64_t kLftjOrderMinUnits = 2048;
+
+static int unit_order(amj_ctx_t* ctx, const LPlan& P, const uint64_t** order) {
+  *order = nullptr;
+  const uint64_t units = lftj_units(P);
+  if (units < kLftjOrderMinUnits) return AMJ_OK;
+  hipStream_t s = ctx->stream;
+  void *keys = nullptr, *alt = nullptr, *tmp = nullptr;
+  AMJ_TRY(scratch(ctx, SLOT_SORT_KEYS, units * 8, &keys));
+  AMJ_TRY(scratch(ctx, SLOT_SORT_ALT, units * 8, &alt));
+  AMJ_TRY(scratch(ctx, SLOT_SORT_TEMP, radix_sort_temp_bytes(units), &tmp));
+  AMJ_HIP_TRY(ctx, launch_lftj_unit_weights(P, (uint64_t*)keys, s));
+  // the weight bytes only: the sort is stable, so equal weights keep unit order
+  AMJ_HIP_TRY(ctx, radix_sort_u64((uint64_t*)keys, (uint64_t*)alt, units, 0xf0u, tmp, s));
+  *order = (const uint64_t*)keys;
+  return AMJ_OK;
+}
+
 static int run_lftj_direct(amj_ctx_t* ctx, const LPlan& P, const amj_opts_t* o,
                            amj_result_t* res) {
   hipStream_t s = ctx->stream;
@@ -122,6 +143,8 @@ static int run_lftj_direct(amj_ctx_t* ctx, const LPlan& P, const amj_opts_t* o,
                            : (uint64_t)((double)sampled * (double)units / (double)su * 1.5) + 65536;
     ++passes;
   }
+  const uint64_t* order = nullptr;
+  AMJ_TRY(unit_order(ctx, P, &order));
   const uint64_t nb = units == 0 ? 1 : (units + kWavesPerBlock - 1) / kWavesPerBlock;
   void *bsum = nullptr, *ctl = nullptr;
   AMJ_TRY(scratch(ctx, SLOT_BLOCK_SUMS, nb * 8, &bsum));
@@ -144,7 +167,7 @@ static int run_lftj_direct(amj_ctx_t* ctx, const LPlan& P, const amj_opts_t* o,
     AMJ_HIP_TRY(ctx, hipMemsetAsync(ctl, 0, 8, s));
     AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
     AMJ_HIP_TRY(ctx, launch_lftj_direct(P, out, capacity, hash, (unsigned long long*)ctl,
-                                        (uint64_t*)bsum, &nb_used, s));
+                                        (uint64_t*)bsum, &nb_used, order, s));
     AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
     AMJ_HIP_TRY(ctx, hipMemcpyAsync(&rows, ctl, 8, hipMemcpyDeviceToHost, s));
     AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
@@ -181,8 +204,10 @@ static int run_lftj_dfs(amj_ctx_t* ctx, const LPlan& P, const amj_opts_t* o, amj
   // opts.ws runs this same engine: every intersection is already spread over the 64 lanes of
   // a wave (what the reference's ws buys by splitting long intersections)
   AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[2], s));
+  const uint64_t* order = nullptr;
+  AMJ_TRY(unit_order(ctx, P, &order));
   AMJ_HIP_TRY(ctx, hipMemsetAsync(counts, 0, (units + 1) * 8, s));
-  AMJ_HIP_TRY(ctx, launch_lftj_count(P, (uint64_t*)counts, s));
+  AMJ_HIP_TRY(ctx, launch_lftj_count(P, (uint64_t*)counts, order, s));
   AMJ_HIP_TRY(ctx, exclusive_scan_u64((uint64_t*)counts, (uint64_t*)counts, units + 1,
                                       (uint64_t*)tmp, s));
   uint64_t total = 0;
@@ -207,7 +232,7 @@ static int run_lftj_dfs(amj_ctx_t* ctx, const LPlan& P, const amj_opts_t* o, amj
   uint64_t nb_used = 0;
   AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
   AMJ_HIP_TRY(ctx, launch_lftj_write(P, (const uint64_t*)counts, out, hash, (uint64_t*)bsum,
-                                     &nb_used, s));
+                                     &nb_used, order, s));
   AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
   if (hash) AMJ_TRY(finish_checksum(ctx, (const uint64_t*)bsum, nb_used, &res->checksum));
   AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[5], s));

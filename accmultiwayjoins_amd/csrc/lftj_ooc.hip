@@ -151,11 +151,11 @@ int run_lftj_ooc(amj_ctx_t* ctx, const LPlan& P, const HostView* hv, const amj_o
       // count + checksum of this combination (the write pass in checksum-only mode)
       uint64_t nb = 0;
       AMJ_HIP_TRY(ctx, hipMemsetAsync(counts, 0, (units + 1) * 8, s));
-      AMJ_HIP_TRY(ctx, launch_lftj_count(Q, (uint64_t*)counts, s));
+      AMJ_HIP_TRY(ctx, launch_lftj_count(Q, (uint64_t*)counts, nullptr, s));
       AMJ_HIP_TRY(ctx, exclusive_scan_u64((uint64_t*)counts, (uint64_t*)counts, units + 1,
                                           (uint64_t*)tmp, s));
       AMJ_HIP_TRY(ctx, launch_lftj_write(Q, (const uint64_t*)counts, LOut{}, true,
-                                         (uint64_t*)bsum, &nb, s));
+                                         (uint64_t*)bsum, &nb, nullptr, s));
       AMJ_HIP_TRY(ctx, reduce_sum_u64((const uint64_t*)bsum, nb, (uint64_t*)csum_dev,
                                       (uint64_t*)tmp, s));
       uint64_t hc[2] = {0, 0};

@@ -223,3 +223,24 @@ def test_dfs_direct_output_same_multiset(ctx, query, cap):
     assert _tuples(r) == _tuples(want)
     if cap:
         assert r.passes == 2  # the first pass overran its output
+
+
+def test_heaviest_units_first_order_same_results(ctx):
+    # R-MAT 18 has 2,716 units of 64 roots: enough for the DFS engine to launch its units in
+    # descending weight order (count, write and direct passes). Results do not depend on it.
+    s, d = G.rmat_oriented(18, 16, seed=42)
+    assert len(np.unique(s)) > 2048 * 64
+    n3, c3 = O.triangles(s, d)
+    r = ctx.lftj_pattern(s, d, "TRI", "DFS", materialize=False)
+    assert (r.count, r.checksum) == (n3, c3)
+    w = ctx.lftj_pattern(s, d, "TRI", "DFS", materialize=True)
+    assert (w.count, w.checksum, w.rows_written) == (n3, c3, n3)
+    md5 = ctx.md5(w)
+    ctx.free(w)
+    x = ctx.lftj_pattern(s, d, "TRI", "DFS", materialize=True, dro=1)
+    assert (x.count, x.checksum, x.rows_written) == (n3, c3, n3)
+    assert ctx.md5(x) == md5  # the same multiset of tuples (an order-free fingerprint)
+    ctx.free(x)
+    h4 = ctx.pattern_join(s, d, "FOUR", materialize=False)
+    l4 = ctx.lftj_pattern(s, d, "FOUR", "DFS", materialize=False)
+    assert (l4.count, l4.checksum) == (h4.count, h4.checksum)
