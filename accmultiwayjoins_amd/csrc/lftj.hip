@@ -158,8 +158,9 @@ __device__ __forceinline__ void descend(const LPlan& P, const LLv<A>& par, Sink&
 
 // Unit weights for the launch order: the squared candidate count at attribute 1, summed over
 // the unit's roots (what a root's second level costs when every candidate opens a list of its
-// own), as the key (~min(weight, 2^32-1) << 32) | unit, so an ascending sort puts the heaviest
-// units first and the dispatcher starts them before the light ones.
+// own), as the key (~bits(float(weight)) << 32) | unit (the bits of a non-negative float order
+// like its value), so an ascending sort puts the heaviest units first and the dispatcher starts
+// them before the light ones.
 template <int A>
 __global__ __launch_bounds__(kBlock) void lftj_unit_weight_kernel(const LPlan P,
                                                                   uint64_t* __restrict__ keys,
@@ -169,13 +170,7 @@ __global__ __launch_bounds__(kBlock) void lftj_unit_weight_kernel(const LPlan P,
   LLv<A> l0;
   level0(P, k, l0);
   const uint64_t w = wave_sum_u64((uint64_t)l0.dlen * (uint64_t)l0.dlen);
-  const uint64_t c = w > 0xffffffffull ? 0xffffffffull : w;
-  if (lane_id() == 0) keys[k] = ((0xffffffffull - c) << 32) | k;
-}
-
-// Unit processed by wave slot k: order[k] holds it in its low 32 bits (nullptr: unit k).
-__device__ __forceinline__ uint64_t unit_of(const uint64_t* order, uint64_t k) {
-  return order ? (uint64_t)(uint32_t)order[k] : k;
+  if (lane_id() == 0) keys[k] = weight_key(w, k);
 }
 
 // Count pass over units k * stride + offset (stride > 1: an evenly strided sample), counts[k];

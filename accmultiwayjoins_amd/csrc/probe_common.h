@@ -195,6 +195,19 @@ __device__ __forceinline__ uint64_t hash_tuple(const int32_t (&v)[kMaxAttrs], in
   return h;
 }
 
+// Sort key of a unit for a heaviest-first launch order: (~bits(float(weight)) << 32) | unit.
+// The bit pattern of a non-negative float orders like its value, so an ascending sort of the
+// keys lists units by descending weight (ties in unit order).
+__device__ __forceinline__ uint64_t weight_key(uint64_t weight, uint64_t unit) {
+  const uint32_t b = __float_as_uint((float)weight);
+  return ((uint64_t)(~b) << 32) | (uint32_t)unit;
+}
+
+// Unit processed by wave slot k: order[k] holds it in its low 32 bits (nullptr: unit k).
+__device__ __forceinline__ uint64_t unit_of(const uint64_t* order, uint64_t k) {
+  return order ? (uint64_t)(uint32_t)order[k] : k;
+}
+
 __device__ __forceinline__ void block_sum_store(uint64_t v, uint64_t* dst) {
   __shared__ uint64_t part[kWavesPerBlock];
   v = wave_sum_u64(v);
