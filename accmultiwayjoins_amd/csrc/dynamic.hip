@@ -13,7 +13,8 @@
 //     `split` and publishes the rest as tasks of `split` children each; a task is the 64-lane
 //     frontier of that level (bindings, runs, prefix) plus its child range, so the wave that
 //     takes it continues exactly where the owner would have;
-//   * results are placed with one atomic reservation per last-level frontier (not per tuple);
+//   * results gather in a per-wave LDS buffer and are placed with one atomic reservation per
+//     full buffer (not per tuple, nor per last-level frontier);
 //   * a wave leaves when no unit is left, the queue is empty and no wave holds work (a wave
 //     holding work is running, so it finishes and, having pushed tasks, takes them itself if
 //     nobody else does). Every wait is bounded: a wave that sees no unit or task completed
@@ -37,12 +38,17 @@ struct DynCtl {
   unsigned long long pushed;     // tasks published
   unsigned long long progress;   // units and tasks completed
   unsigned int busy;             // waves holding a unit or a task
-  unsigned int flags;            // 1: a wait hit the time limit; 2: output capacity overrun
+  unsigned int flags;            // 1: a wait hit the time limit
 };
 
 constexpr uint64_t kMaxClaim = 64;  // units claimed at once by a wave
 constexpr uint32_t kFlagSpin = 1u;
-constexpr uint32_t kFlagOverflow = 2u;
+
+// Per-wave state of the persistent engine besides the accumulators.
+struct DynWave {
+  DirectStage st;  // staged output rows (materialising passes)
+  bool qfull;      // a task reservation failed: the queue has no room left in this launch
+};
 
 // Task record: 16 header words then 10 words per lane (bindings, lo, len, excl).
 constexpr int kTaskHdr = 16;
@@ -60,19 +66,17 @@ struct DynArgs {
   uint64_t stall_ticks;  // wall-clock ticks without any completed work before a wave gives up
 };
 
-__device__ __forceinline__ unsigned long long ld_acq(const unsigned long long* p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ unsigned int ld_acq(const unsigned int* p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-// polling reads: relaxed, so idle waves do not invalidate the caches of the working ones
+// Every access to the control block is relaxed: an acquire at agent scope invalidates the
+// CU's vector caches, and idle waves polling with acquires (or workers finishing chunks with
+// acquire-release updates) evicted the table runs the working waves on the same CU were
+// reading — ten times slower on FOUR. No data is published through these counters: a task
+// record is published by its own ready flag (release store, acquire by the taker), and a wave
+// that leaves early on a stale view only gives up parallelism (a pusher serves its own tasks).
 __device__ __forceinline__ unsigned long long ld_rlx(const unsigned long long* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-template <typename T>
-__device__ __forceinline__ T add_acq_rel(T* p, T v) {
-  return __hip_atomic_fetch_add(p, v, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+__device__ __forceinline__ unsigned int ld_rlx(const unsigned int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 template <typename T>
 __device__ __forceinline__ T add_relaxed(T* p, T v) {
@@ -133,7 +137,7 @@ __device__ __forceinline__ uint64_t reserve_tasks(const DynArgs& A, uint64_t k, 
       if (tl >= A.cap) break;
       const unsigned long long room = A.cap - tl;
       const unsigned long long want = k < room ? k : room;
-      if (__hip_atomic_compare_exchange_strong(&A.ctl->q_tail, &tl, tl + want, __ATOMIC_ACQ_REL,
+      if (__hip_atomic_compare_exchange_strong(&A.ctl->q_tail, &tl, tl + want, __ATOMIC_RELAXED,
                                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
         first = tl;
         n = want;
@@ -147,7 +151,7 @@ __device__ __forceinline__ uint64_t reserve_tasks(const DynArgs& A, uint64_t k, 
 
 template <int L, int t, int MODE>
 __device__ __forceinline__ void dyn_descend(const Plan& P, const Lv& cur, uint64_t c0, uint64_t c1,
-                            const ProbeOut& out, const DynArgs& A, Acc& acc) {
+                            const ProbeOut& out, const DynArgs& A, Acc& acc, DynWave& W) {
   if constexpr (t == L - 1) {
     // the last level is never split: its children are the results, [0, T)
     if constexpr (MODE == MODE_COUNT) {
@@ -156,20 +160,12 @@ __device__ __forceinline__ void dyn_descend(const Plan& P, const Lv& cur, uint64
       emit<false, mode_hash<MODE>()>(P, t, cur, 0, out, acc);
       acc.count += cur.T;
     } else {
-      uint64_t base = 0;
-      if (lane_id() == 0) base = add_relaxed(&A.ctl->out_rows, (unsigned long long)cur.T);
-      base = lane0_u64(base);
-      if (base + cur.T <= A.out_cap) {
-        emit<mode_mat<MODE>(), mode_hash<MODE>()>(P, t, cur, base, out, acc);
-      } else if (lane_id() == 0) {
-        __hip_atomic_fetch_or(&A.ctl->flags, kFlagOverflow, __ATOMIC_RELAXED,
-                              __HIP_MEMORY_SCOPE_AGENT);
-      }
+      direct_emit<true, mode_hash<MODE>()>(P, t, cur, out, acc, W.st, &A.ctl->out_rows);
       acc.count += cur.T;
     }
   } else {
     for (uint64_t c = c0; c < c1; c += kWave) {
-      if (A.split && c1 - c > A.split + (uint64_t)kWave) {
+      if (A.split && !W.qfull && c1 - c > A.split + (uint64_t)kWave) {
         // keep [c, c + split), publish the remaining pieces (the last ones if the queue is short)
         const uint64_t keep_end = c + A.split;
         const uint64_t k = (c1 - keep_end + A.split - 1) / A.split;
@@ -184,11 +180,13 @@ __device__ __forceinline__ void dyn_descend(const Plan& P, const Lv& cur, uint64
           }
           if (lane_id() == 0) add_relaxed(&A.ctl->pushed, (unsigned long long)got);
           c1 = keep_end + first_piece * A.split;
+        } else {
+          W.qfull = true;  // slots are never reused within a launch: stop asking
         }
       }
       Lv ch;
       make_child<t>(P, cur, c, c1, ch);
-      if (ch.T) dyn_descend<L, t + 1, MODE>(P, ch, 0, ch.T, out, A, acc);
+      if (ch.T) dyn_descend<L, t + 1, MODE>(P, ch, 0, ch.T, out, A, acc, W);
     }
   }
 }
@@ -200,7 +198,7 @@ __device__ __forceinline__ int64_t try_pop(const DynArgs& A) {
     unsigned long long h = ld_rlx(&A.ctl->q_head);
     const unsigned long long tl = ld_rlx(&A.ctl->q_tail);
     if (h < tl && __hip_atomic_compare_exchange_strong(&A.ctl->q_head, &h, h + 1,
-                                                       __ATOMIC_ACQ_REL, __ATOMIC_RELAXED,
+                                                       __ATOMIC_RELAXED, __ATOMIC_RELAXED,
                                                        __HIP_MEMORY_SCOPE_AGENT)) {
       r = (int64_t)h;
       const uint64_t t0 = wall_clock64();
@@ -224,6 +222,10 @@ __device__ __forceinline__ int64_t try_pop(const DynArgs& A) {
 template <int L, int MODE>
 __global__ __launch_bounds__(kBlock) void dyn_kernel(const Plan P, const ProbeOut out,
                                                      const DynArgs A) {
+  constexpr bool MAT = mode_mat<MODE>();
+  __shared__ int32_t stage_mem[MAT ? kWavesPerBlock * kMaxAttrs * kDirectRows : 1];
+  DynWave W{{(lds_i32*)&stage_mem[MAT ? (threadIdx.x / kWave) * kMaxAttrs * kDirectRows : 0], 0u},
+            false};
   Acc acc{};
   uint64_t seen = ~0ull, since = 0;  // progress counter last seen, and when it last moved
   for (;;) {
@@ -251,11 +253,11 @@ __global__ __launch_bounds__(kBlock) void dyn_kernel(const Plan P, const ProbeOu
       for (uint64_t v = u; v < ue; ++v) {
         Lv l0;
         make_level0(P, v * kWave, l0);
-        if (l0.T) dyn_descend<L, 0, MODE>(P, l0, 0, l0.T, out, A, acc);
+        if (l0.T) dyn_descend<L, 0, MODE>(P, l0, 0, l0.T, out, A, acc, W);
       }
       if (lane_id() == 0) {
         add_relaxed(&A.ctl->progress, 1ull);
-        add_acq_rel(&A.ctl->busy, ~0u);
+        add_relaxed(&A.ctl->busy, ~0u);
       }
       continue;
     }
@@ -268,18 +270,18 @@ __global__ __launch_bounds__(kBlock) void dyn_kernel(const Plan P, const ProbeOu
       Lv l;
       read_task(A, (uint64_t)slot, t, c0, c1, l);
       sfor<0, L - 1>([&](auto tt) {
-        if ((int)tt == t) dyn_descend<L, tt, MODE>(P, l, c0, c1, out, A, acc);
+        if ((int)tt == t) dyn_descend<L, tt, MODE>(P, l, c0, c1, out, A, acc, W);
       });
       if (lane_id() == 0) {
         add_relaxed(&A.ctl->progress, 1ull);
-        add_acq_rel(&A.ctl->busy, ~0u);
+        add_relaxed(&A.ctl->busy, ~0u);
       }
       continue;
     }
     if (slot == -2) break;
     uint32_t done = 0;
     if (lane_id() == 0)
-      done = (ld_acq(&A.ctl->busy) == 0u && ld_rlx(&A.ctl->q_head) >= ld_rlx(&A.ctl->q_tail))
+      done = (ld_rlx(&A.ctl->busy) == 0u && ld_rlx(&A.ctl->q_head) >= ld_rlx(&A.ctl->q_tail))
                  ? 1u : 0u;
     if (uniform(__shfl(done, 0, kWave))) break;
     // no work here while other waves hold some: give up only if nothing completes for long
@@ -295,6 +297,7 @@ __global__ __launch_bounds__(kBlock) void dyn_kernel(const Plan P, const ProbeOu
     }
     __builtin_amdgcn_s_sleep(64);
   }
+  if (MAT) direct_flush(out, P.n_attrs, W.st, &A.ctl->out_rows);
   const uint64_t cs = wave_sum_u64(acc.csum);
   if (lane_id() == 0) {
     add_relaxed(&A.ctl->count, (unsigned long long)acc.count);
@@ -314,7 +317,8 @@ __global__ __launch_bounds__(kBlock) void dyn_sample_kernel(const Plan P, uint64
   make_level0(P, u * kWave, l0);
   Acc acc{};
   const DynArgs none{};  // no sharing, no output
-  if (l0.T) dyn_descend<L, 0, MODE_COUNT>(P, l0, 0, l0.T, ProbeOut{}, none, acc);
+  DynWave W{{nullptr, 0u}, true};
+  if (l0.T) dyn_descend<L, 0, MODE_COUNT>(P, l0, 0, l0.T, ProbeOut{}, none, acc, W);
   if (lane_id() == 0) add_relaxed(sum, (unsigned long long)acc.count);
 }
 
@@ -349,13 +353,36 @@ hipError_t launch_dyn_sample(const Plan& P, uint64_t samples, unsigned long long
   return hipGetLastError();
 }
 
-int dyn_occupancy_grid(const Plan& P, int cus) {
+// Persistent grid: as many workgroups as fit on the device at once for the instantiation that
+// runs (its registers and, when materialising, its staging buffer bound the occupancy).
+template <int L, int MODE>
+static int occupancy_of() {
   int per_cu = 0;
-  // every instantiation has the same launch bounds; the 5-level write kernel is the largest
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dyn_kernel<kMaxLevels, MODE_WRITE_HASH>,
-                                                   kBlock, 0) != hipSuccess || per_cu <= 0)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, dyn_kernel<L, MODE>, kBlock, 0) !=
+          hipSuccess || per_cu <= 0)
     per_cu = 2;
-  (void)P;
+  return per_cu;
+}
+
+int dyn_occupancy_grid(const Plan& P, bool mat, bool hash, int cus) {
+  const int mode = mat ? (hash ? MODE_WRITE_HASH : MODE_WRITE) : (hash ? MODE_HASH : MODE_COUNT);
+  int per_cu = 2;
+#define OCC_M(LV, M) per_cu = occupancy_of<LV, M>()
+#define OCC(LV)                                                \
+  if (mode == MODE_WRITE) OCC_M(LV, MODE_WRITE);                \
+  else if (mode == MODE_WRITE_HASH) OCC_M(LV, MODE_WRITE_HASH); \
+  else if (mode == MODE_HASH) OCC_M(LV, MODE_HASH);             \
+  else OCC_M(LV, MODE_COUNT)
+  switch (P.n_levels) {
+    case 1: OCC(1); break;
+    case 2: OCC(2); break;
+    case 3: OCC(3); break;
+    case 4: OCC(4); break;
+    case 5: OCC(5); break;
+    default: break;
+  }
+#undef OCC
+#undef OCC_M
   return per_cu * cus;
 }
 
@@ -406,15 +433,17 @@ int run_dynamic(amj_ctx_t* ctx, Plan& P, const amj_opts_t* o, amj_result_t* res)
     A.tasks = (uint32_t*)tasks;
     A.ready = (uint32_t*)ready;
   }
+  const bool mat = o->materialize != 0;
+  // sized for the final pass (a count pass before it fits at least as many waves)
   const unsigned grid = (unsigned)std::max<uint64_t>(
       1, std::min<uint64_t>((units + kWavesPerBlock - 1) / kWavesPerBlock,
-                            (uint64_t)dyn_occupancy_grid(P, ctx->props.multiProcessorCount)));
+                            (uint64_t)dyn_occupancy_grid(P, mat, o->checksum || !mat,
+                                                         ctx->props.multiProcessorCount)));
   AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[2], s));
   DynCtl hc{};
   double kms = 0.0;
   uint64_t capacity = 0;
   uint32_t passes = 0;
-  const bool mat = o->materialize != 0;
   if (!o->dro || !mat) {
     // count pass, then an exact allocation (WS without DRO, AMHJ_WS.cuh:478-557)
     if (mat) {
@@ -455,12 +484,12 @@ int run_dynamic(amj_ctx_t* ctx, Plan& P, const amj_opts_t* o, amj_result_t* res)
     const bool hash = o->checksum || !mat;
     AMJ_TRY(dyn_pass(ctx, P, out, hash, A, grid, &hc, &kms));
     ++passes;
-    if (!(hc.flags & kFlagOverflow)) break;
+    if (hc.out_rows <= capacity) break;
     // the estimate was short: the pass counted every result, rerun into an exact allocation
     amj_result_free(ctx, res);
     capacity = hc.out_rows;
   }
-  if (hc.flags & kFlagOverflow) {
+  if (hc.out_rows > capacity) {
     set_error(ctx, AMJ_ERR_STATE, "direct output overran an exact allocation");
     return AMJ_ERR_STATE;
   }
@@ -582,13 +611,14 @@ int run_ooc(amj_ctx_t* ctx, Plan& P, const amj_opts_t* o, amj_result_t* res) {
   AMJ_TRY(scratch(ctx, SLOT_OOC_A, slice * nc * 4, &buf[0]));
   AMJ_TRY(scratch(ctx, SLOT_OOC_B, slice * nc * 4, &buf[1]));
   int32_t* bufs[2] = {(int32_t*)buf[0], (int32_t*)buf[1]};
+  const bool mat = o->materialize != 0;
   const unsigned grid = (unsigned)std::max<uint64_t>(
       1, std::min<uint64_t>((slice / kWave + kWavesPerBlock - 1) / kWavesPerBlock,
-                            (uint64_t)dyn_occupancy_grid(P, ctx->props.multiProcessorCount)));
+                            (uint64_t)dyn_occupancy_grid(P, mat, o->checksum || !mat,
+                                                         ctx->props.multiProcessorCount)));
   AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[2], s));
   DynCtl hc{};
   uint64_t n_slices = 0;
-  const bool mat = o->materialize != 0;
   uint32_t passes = 0;
   ProbeOut out{};
   if (mat) {  // counting stream, then the exact allocation
@@ -613,7 +643,7 @@ int run_ooc(amj_ctx_t* ctx, Plan& P, const amj_opts_t* o, amj_result_t* res) {
   const bool hash = o->checksum || !mat;
   AMJ_TRY(ooc_stream(ctx, P, out, hash, A, slice, bufs, grid, true, &hc, &n_slices));
   ++passes;
-  if (hc.flags & kFlagOverflow) {
+  if (hc.out_rows > out.capacity) {
     set_error(ctx, AMJ_ERR_STATE, "out-of-core probe overran its exact allocation");
     return AMJ_ERR_STATE;
   }

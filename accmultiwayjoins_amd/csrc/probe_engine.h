@@ -110,6 +110,75 @@ __device__ __forceinline__ void emit(const Plan& P, int t, const Lv& l,
   }
 }
 
+// Direct output with dynamic placement: a wave gathers its results in an LDS buffer of
+// kDirectRows rows and reserves room in the output with one atomic per full buffer, so the
+// output stays dense and the shared row counter sees one update per kDirectRows results (tiny
+// last-level frontiers, as in TRI and FOUR, would otherwise make one update per few results).
+// Rows past the output's capacity are not written; the reserved total then tells the host that
+// the pass must run again into an exact allocation.
+constexpr uint32_t kDirectRows = 256;
+typedef __attribute__((address_space(3))) int32_t lds_i32;
+
+struct DirectStage {
+  lds_i32* buf;   // this wave's rows: attribute a of row i at buf[a * kDirectRows + i]
+  uint32_t fill;  // rows held (uniform)
+};
+
+__device__ __forceinline__ void direct_flush(const ProbeOut& out, int na, DirectStage& st,
+                                             unsigned long long* rows) {
+  if (st.fill == 0) return;
+  unsigned long long r0 = 0;
+  if (lane_id() == 0)
+    r0 = __hip_atomic_fetch_add(rows, (unsigned long long)st.fill, __ATOMIC_RELAXED,
+                                __HIP_MEMORY_SCOPE_AGENT);
+  const uint64_t base = uniform((uint64_t)r0);
+  __asm__ __volatile__("" ::: "memory");  // LDS ops of one wave complete in order
+  for (uint32_t i = (uint32_t)lane_id(); i < st.fill; i += kWave) {
+    const uint64_t pos = base + i;
+    if (pos < out.capacity)
+      sfor<0, kMaxAttrs>([&](auto a) {
+        if (a < na) out.cols[a][pos] = st.buf[a * kDirectRows + i];
+      });
+  }
+  __asm__ __volatile__("" ::: "memory");
+  st.fill = 0;
+}
+
+// Results [0, T) of a last level into the wave's buffer (MAT) and the checksum (HASH).
+template <bool MAT, bool HASH>
+__device__ __forceinline__ void direct_emit(const Plan& P, int t, const Lv& l,
+                                            const ProbeOut& out, Acc& acc, DirectStage& st,
+                                            unsigned long long* rows) {
+  const bool bind = P.val_mode[t] == VAL_BIND;
+  const int vattr = P.val_attr[t];
+  const int32_t* vals = P.tab[t].vals;
+  const uint64_t e1 = l.T;
+  for (uint64_t c = 0; c < e1; c += kWave) {
+    const uint64_t g = c + (uint64_t)lane_id();
+    const bool valid = g < e1;
+    const uint64_t gg = valid ? g : e1 - 1;
+    const int p = parent_lane(l, c, gg);
+    const uint32_t plo = __shfl(l.lo, p, kWave);
+    const uint32_t pex = (uint32_t)__shfl((uint32_t)l.excl, p, kWave);
+    const uint32_t row = plo + ((uint32_t)gg - pex);
+    int32_t v[kMaxAttrs];
+    sfor<0, kMaxAttrs>([&](auto a) { v[a] = (a < P.n_attrs) ? __shfl(l.b[a], p, kWave) : 0; });
+    if (bind) set_attr(v, vattr, vals[row]);
+    if (HASH && valid) acc.csum += hash_tuple(v, P.n_attrs);
+    if (MAT) {
+      const uint32_t m = e1 - c < (uint64_t)kWave ? (uint32_t)(e1 - c) : (uint32_t)kWave;
+      if (st.fill + m > kDirectRows) direct_flush(out, P.n_attrs, st, rows);
+      __asm__ __volatile__("" ::: "memory");
+      if (valid)
+        sfor<0, kMaxAttrs>([&](auto a) {
+          if (a < P.n_attrs) st.buf[a * kDirectRows + st.fill + lane_id()] = v[a];
+        });
+      __asm__ __volatile__("" ::: "memory");
+      st.fill += m;
+    }
+  }
+}
+
 // Depth-first over the frontier levels of one unit (count, or write at acc.out_base).
 template <int L, int t, int MODE>
 __device__ __forceinline__ void descend(const Plan& P, const Lv& cur,
