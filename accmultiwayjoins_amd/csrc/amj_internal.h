@@ -243,9 +243,11 @@ struct ProbeOut {
 hipError_t launch_probe_count(const Plan& plan, uint64_t* unit_counts, hipStream_t s);
 // Write pass: units write at unit_base[u] (exclusive scan of counts); per-block checksum
 // partials go to block_sums. hash=false skips the checksum when materialising.
+// With unit_base == nullptr (no output) it is the only pass: per-block result counts go to
+// block_counts.
 hipError_t launch_probe_write(const Plan& plan, const uint64_t* unit_base, const ProbeOut& out,
                               bool hash, uint64_t* block_sums, uint64_t* n_blocks_out,
-                              hipStream_t s);
+                              hipStream_t s, uint64_t* block_counts = nullptr);
 uint64_t probe_units(const Plan& plan);
 uint64_t probe_write_blocks(const Plan& plan);
 

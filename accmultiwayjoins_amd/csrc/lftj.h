@@ -62,9 +62,12 @@ hipError_t launch_lftj_count(const LPlan& P, uint64_t* unit_counts, const uint64
 struct LOut {
   int32_t* c[kMaxLftjAttrs];  // result columns (c[0] == nullptr: checksum only)
 };
+// With unit_base == nullptr (no output) it is the only pass: per-block result counts go to
+// block_counts.
 hipError_t launch_lftj_write(const LPlan& P, const uint64_t* unit_base, const LOut& cols,
                              bool hash, uint64_t* block_sums, uint64_t* n_blocks,
-                             const uint64_t* order, hipStream_t s);
+                             const uint64_t* order, hipStream_t s,
+                             uint64_t* block_counts = nullptr);
 // Counts of `units` units k * stride + offset (an evenly strided sample when stride > 1).
 hipError_t launch_lftj_count(const LPlan& P, uint64_t* counts, hipStream_t s, uint64_t units,
                              uint64_t stride, uint64_t offset);

@@ -191,23 +191,30 @@ __global__ __launch_bounds__(kBlock) void lftj_count_kernel(const LPlan P,
   if (lane_id() == 0) counts[order ? unit : k] = sink.count;
 }
 
+// Write pass at the scanned bases; or, with base == nullptr (nothing materialised), the only
+// pass: it hashes every result and leaves per-block result counts in block_counts.
 template <int A, int MODE>
 __global__ __launch_bounds__(kBlock) void lftj_write_kernel(const LPlan P,
                                                             const uint64_t* __restrict__ base,
                                                             const LOut cols,
                                                             uint64_t* __restrict__ block_sums,
+                                                            uint64_t* __restrict__ block_counts,
                                                             uint64_t units,
                                                             const uint64_t* __restrict__ order) {
   const uint64_t k = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
   WriteSink<MODE> sink{cols, 0, 0};
+  uint64_t start = 0;
   if (k < units) {
     const uint64_t unit = unit_of(order, k);
     LLv<A> l0;
     level0(P, unit, l0);
-    sink.out_base = uniform(base[unit]);
+    start = base ? uniform(base[unit]) : 0;
+    sink.out_base = start;
     if (l0.T) descend<A, 1>(P, l0, sink);
   }
-  block_sum_store(sink.csum, &block_sums[blockIdx.x]);
+  // out_base is uniform: lane 0 carries the wave's count
+  block_sum_store2(sink.csum, lane_id() == 0 ? sink.out_base - start : 0,
+                   &block_sums[blockIdx.x], block_counts ? &block_counts[blockIdx.x] : nullptr);
 }
 
 template <int A, bool HASH>
@@ -295,7 +302,7 @@ hipError_t launch_lftj_direct(const LPlan& P, const LOut& cols, uint64_t capacit
 
 hipError_t launch_lftj_write(const LPlan& P, const uint64_t* base, const LOut& cols,
                              bool hash, uint64_t* block_sums, uint64_t* n_blocks,
-                             const uint64_t* order, hipStream_t s) {
+                             const uint64_t* order, hipStream_t s, uint64_t* block_counts) {
   const uint64_t units = lftj_units(P);
   const uint64_t blocks = units == 0 ? 1 : (units + kWavesPerBlock - 1) / kWavesPerBlock;
   *n_blocks = blocks;
@@ -303,7 +310,7 @@ hipError_t launch_lftj_write(const LPlan& P, const uint64_t* base, const LOut& c
   const int mode = cols.c[0] ? (hash ? MODE_WRITE_HASH : MODE_WRITE) : MODE_HASH;
 #define LW(AV, M) \
   hipLaunchKernelGGL((lftj_write_kernel<AV, M>), grid, dim3(kBlock), 0, s, P, base, cols, block_sums, \
-                     units, order)
+                     block_counts, units, order)
 #define LWA(AV)                                              \
   if (mode == MODE_WRITE) LW(AV, MODE_WRITE);                \
   else if (mode == MODE_WRITE_HASH) LW(AV, MODE_WRITE_HASH); \

@@ -194,8 +194,38 @@ static int run_lftj_direct(amj_ctx_t* ctx, const LPlan& P, const amj_opts_t* o,
   return AMJ_OK;
 }
 
+// Count + checksum without output: one pass that hashes every result and counts per block.
+static int run_lftj_count(amj_ctx_t* ctx, const LPlan& P, amj_result_t* res) {
+  hipStream_t s = ctx->stream;
+  const uint64_t units = lftj_units(P);
+  const uint64_t nb = units == 0 ? 1 : (units + kWavesPerBlock - 1) / kWavesPerBlock;
+  void *bsum = nullptr, *bcnt = nullptr;
+  AMJ_TRY(scratch(ctx, SLOT_BLOCK_SUMS, nb * 8, &bsum));
+  AMJ_TRY(scratch(ctx, SLOT_MISC1, nb * 8, &bcnt));
+  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[2], s));
+  const uint64_t* order = nullptr;
+  AMJ_TRY(unit_order(ctx, P, &order));
+  uint64_t nb_used = 0;
+  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
+  AMJ_HIP_TRY(ctx, launch_lftj_write(P, nullptr, LOut{}, true, (uint64_t*)bsum, &nb_used, order,
+                                     s, (uint64_t*)bcnt));
+  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
+  AMJ_TRY(finish_checksum(ctx, (const uint64_t*)bsum, nb_used, &res->checksum));
+  AMJ_TRY(finish_checksum(ctx, (const uint64_t*)bcnt, nb_used, &res->count));
+  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[5], s));
+  AMJ_HIP_TRY(ctx, hipEventSynchronize(ctx->ev[5]));
+  res->probe_ms = elapsed_ms(ctx->ev[2], ctx->ev[5]);
+  res->probe_kernel_ms = elapsed_ms(ctx->ev[3], ctx->ev[4]);
+  res->probe_kernel_launches = 1;
+  res->rows_written = 0;
+  res->passes = 1;
+  res->path = AMJ_PATH_LFTJ_DFS;
+  return AMJ_OK;
+}
+
 static int run_lftj_dfs(amj_ctx_t* ctx, const LPlan& P, const amj_opts_t* o, amj_result_t* res) {
   if (o->dro && o->materialize) return run_lftj_direct(ctx, P, o, res);
+  if (!o->materialize) return run_lftj_count(ctx, P, res);
   hipStream_t s = ctx->stream;
   const uint64_t units = lftj_units(P);
   void *counts = nullptr, *tmp = nullptr, *bsum = nullptr;

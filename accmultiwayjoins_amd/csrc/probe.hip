@@ -40,21 +40,26 @@ __global__ __launch_bounds__(kBlock) void probe_count_kernel(const Plan P,
   if (lane_id() == 0) counts[unit] = acc.count;
 }
 
+// Write pass at the scanned bases; or, with base == nullptr (nothing materialised), the only
+// pass: it hashes every result and leaves per-block result counts in block_counts.
 template <int L, int MODE>
 __global__ __launch_bounds__(kBlock) void probe_write_kernel(const Plan P,
                                                              const uint64_t* __restrict__ base,
                                                              const ProbeOut out,
                                                              uint64_t* __restrict__ block_sums,
+                                                             uint64_t* __restrict__ block_counts,
                                                              uint64_t units) {
   const uint64_t unit = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
   Acc acc{};
   if (unit < units) {
     Lv l0;
     make_level0(P, unit * kWave, l0);
-    acc.out_base = uniform(base[unit]);
+    acc.out_base = base ? uniform(base[unit]) : 0;
     if (l0.T) descend<L, 0, MODE>(P, l0, out, acc);
   }
-  block_sum_store(acc.csum, &block_sums[blockIdx.x]);
+  // acc.count is uniform: lane 0 carries the wave's
+  block_sum_store2(acc.csum, lane_id() == 0 ? acc.count : 0, &block_sums[blockIdx.x],
+                   block_counts ? &block_counts[blockIdx.x] : nullptr);
 }
 
 uint64_t probe_units(const Plan& P) { return (P.probe_n + kWave - 1) / kWave; }
@@ -76,14 +81,14 @@ hipError_t launch_probe_count(const Plan& P, uint64_t* counts, hipStream_t s) {
 
 hipError_t launch_probe_write(const Plan& P, const uint64_t* base, const ProbeOut& out,
                               bool hash, uint64_t* block_sums, uint64_t* n_blocks_out,
-                              hipStream_t s) {
+                              hipStream_t s, uint64_t* block_counts) {
   const uint64_t units = probe_units(P);
   const unsigned grid = (unsigned)probe_write_blocks(P);
   *n_blocks_out = grid;
   const int mode = write_mode(out, hash);
 #define LAUNCH_M(LV, M)                                                                 \
   hipLaunchKernelGGL((probe_write_kernel<LV, M>), dim3(grid), dim3(kBlock), 0, s, P, base, \
-                     out, block_sums, units)
+                     out, block_sums, block_counts, units)
 #define LAUNCH(LV)                                                 \
   if (mode == MODE_WRITE) LAUNCH_M(LV, MODE_WRITE);                \
   else if (mode == MODE_WRITE_HASH) LAUNCH_M(LV, MODE_WRITE_HASH); \

@@ -221,6 +221,29 @@ __device__ __forceinline__ void block_sum_store(uint64_t v, uint64_t* dst) {
   }
 }
 
+// Block sums of two per-lane values with one barrier (dst_b may be nullptr).
+__device__ __forceinline__ void block_sum_store2(uint64_t a, uint64_t b, uint64_t* dst_a,
+                                                 uint64_t* dst_b) {
+  __shared__ uint64_t part2[2][kWavesPerBlock];
+  a = wave_sum_u64(a);
+  b = wave_sum_u64(b);
+  if (lane_id() == 0) {
+    part2[0][threadIdx.x / kWave] = a;
+    part2[1][threadIdx.x / kWave] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t sa = 0, sb = 0;
+#pragma unroll
+    for (int i = 0; i < kWavesPerBlock; ++i) {
+      sa += part2[0][i];
+      sb += part2[1][i];
+    }
+    *dst_a = sa;
+    if (dst_b) *dst_b = sb;
+  }
+}
+
 #define AMJ_DISPATCH_L(P, MACRO)          \
   switch ((P).n_levels) {                 \
     case 1: MACRO(1); break;              \

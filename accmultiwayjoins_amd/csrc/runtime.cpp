@@ -417,7 +417,34 @@ uint64_t budget_left(amj_ctx_t* ctx, const amj_opts_t* o) {
 }
 
 // Plain AMHJ: count pass -> exclusive scan -> write pass (AMHJ.cuh:314-377).
+// Count + checksum without output: one pass that hashes every result and counts per block.
+static int run_plain_count(amj_ctx_t* ctx, Plan& P, amj_result_t* res) {
+  hipStream_t s = ctx->stream;
+  const uint64_t nb = probe_write_blocks(P);
+  void *bsum = nullptr, *bcnt = nullptr;
+  AMJ_TRY(scratch(ctx, SLOT_BLOCK_SUMS, nb * 8, &bsum));
+  AMJ_TRY(scratch(ctx, SLOT_MISC1, nb * 8, &bcnt));
+  uint64_t nb_used = 0;
+  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[2], s));
+  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
+  AMJ_HIP_TRY(ctx, launch_probe_write(P, nullptr, ProbeOut{}, true, (uint64_t*)bsum, &nb_used, s,
+                                      (uint64_t*)bcnt));
+  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
+  AMJ_TRY(finish_checksum(ctx, (const uint64_t*)bsum, nb_used, &res->checksum));
+  AMJ_TRY(finish_checksum(ctx, (const uint64_t*)bcnt, nb_used, &res->count));
+  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[5], s));
+  AMJ_HIP_TRY(ctx, hipEventSynchronize(ctx->ev[5]));
+  res->probe_ms = elapsed_ms(ctx->ev[2], ctx->ev[5]);
+  res->probe_kernel_ms = elapsed_ms(ctx->ev[3], ctx->ev[4]);
+  res->probe_kernel_launches = 1;
+  res->rows_written = 0;
+  res->path = AMJ_PATH_PLAIN;
+  res->passes = 1;
+  return AMJ_OK;
+}
+
 static int run_plain(amj_ctx_t* ctx, Plan& P, const amj_opts_t* o, amj_result_t* res) {
+  if (!o->materialize) return run_plain_count(ctx, P, res);
   hipStream_t s = ctx->stream;
   const uint64_t units = probe_units(P);
   void *counts = nullptr, *tmp = nullptr, *bsum = nullptr;

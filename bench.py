@@ -272,8 +272,8 @@ def ooc_leg(G, rows, budget_gb):
 
 
 def four_leg(G, scale):
-    """4-clique query (FOUR) on an R-MAT graph through AMHJ (amj_execute_pattern, ws=1) and ALFTJ
-    DFS over one trie aliased six times; count + checksum (at scale 20 the ~10^10 results do not
+    """4-clique query (FOUR) on an R-MAT graph through AMHJ (amj_execute_pattern) and ALFTJ DFS
+    over one trie aliased six times; count + checksum (at scale 20 the ~10^10 results do not
     fit one GPU's memory as tuples). The two engines must agree."""
     from accmultiwayjoins_amd import Context
     ctx = Context(0)
@@ -283,8 +283,7 @@ def four_leg(G, scale):
     t = ctx.trie_from_relation(rel)
     four = [(0, 1), (1, 2), (2, 3), (0, 3), (0, 2), (1, 3)]
     res = {}
-    for name, fn in (("amhj", lambda: ctx.execute_pattern(rel, ht, "FOUR", materialize=False,
-                                                          ws=1)),
+    for name, fn in (("amhj", lambda: ctx.execute_pattern(rel, ht, "FOUR", materialize=False)),
                      ("alftj_dfs", lambda: ctx.lftj([t] * 6, [0, 1, 2, 3], "DFS",
                                                     trie_attrs=four, materialize=False))):
         t0 = time.perf_counter()
@@ -302,8 +301,7 @@ def four_leg(G, scale):
     rel.close()
     ctx.close()
     return {"count": ca, "checksum": "%#018x" % sa, "edges": len(src),
-            "amhj": {"value": ca / (pa / 1e3), "unit": "tuples/s", "probe_ms": round(pa, 3),
-                     "opts": "ws=1"},
+            "amhj": {"value": ca / (pa / 1e3), "unit": "tuples/s", "probe_ms": round(pa, 3)},
             "alftj_dfs": {"value": cl / (pl / 1e3), "unit": "tuples/s", "probe_ms": round(pl, 3)},
             "workload": f"FOUR (4-clique) on R-MAT scale {scale} edge factor 16, {len(src)} "
                         f"oriented edges, count + checksum (hash table / trie built untimed)",
