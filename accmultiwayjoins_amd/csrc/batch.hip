@@ -89,14 +89,31 @@ __device__ __forceinline__ void descend_direct(const Plan& P, const Lv& cur, con
   }
 }
 
+// Weights of the launch's units for a heaviest-first order (squared first-table run lengths
+// of each unit's probe rows), as sort keys over the launch's unit offsets w.
+__global__ __launch_bounds__(kBlock) void batch_weight_kernel(const BatchQ* __restrict__ qs,
+                                                              uint32_t nq, uint64_t unit_begin,
+                                                              uint64_t total,
+                                                              uint64_t* __restrict__ keys) {
+  const uint64_t w = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+  if (w >= total) return;  // no barriers in this kernel
+  const uint64_t g = unit_begin + w;
+  const uint32_t q = uniform(query_of(qs, nq, g));
+  const BatchQ& Q = qs[q];
+  Lv l0;
+  make_level0(Q.P, (g - Q.unit0) * kWave, l0);
+  const uint64_t wt = wave_sum_u64((uint64_t)l0.len * (uint64_t)l0.len);
+  if (lane_id() == 0) keys[w] = weight_key(wt, w);
+}
+
 template <int L, bool MAT>
 __global__ __launch_bounds__(kBlock) void batch_direct_kernel(
     const BatchQ* __restrict__ qs, uint32_t nq, uint64_t unit_begin, uint64_t total,
     unsigned long long* __restrict__ rows, unsigned long long* __restrict__ counts,
-    unsigned long long* __restrict__ csums) {
-  const uint64_t w = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
-  if (w >= total) return;  // no barriers in this kernel
-  const uint64_t g = unit_begin + w;
+    unsigned long long* __restrict__ csums, const uint64_t* __restrict__ order) {
+  const uint64_t k = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+  if (k >= total) return;  // no barriers in this kernel
+  const uint64_t g = unit_begin + unit_of(order, k);
   const uint32_t q = uniform(query_of(qs, nq, g));
   const BatchQ& Q = qs[q];
   __shared__ int32_t stage_mem[MAT ? kWavesPerBlock * kMaxAttrs * kDirectRows : 1];
@@ -167,10 +184,18 @@ hipError_t launch_batch_write(int L, const void* qs, uint32_t nq, uint64_t unit_
   return hipGetLastError();
 }
 
+hipError_t launch_batch_weights(const void* qs, uint32_t nq, uint64_t unit_begin,
+                                uint64_t total, uint64_t* keys, hipStream_t s) {
+  if (total == 0) return hipSuccess;
+  hipLaunchKernelGGL(batch_weight_kernel, dim3(grid_units(total)), dim3(kBlock), 0, s,
+                     (const BatchQ*)qs, nq, unit_begin, total, keys);
+  return hipGetLastError();
+}
+
 hipError_t launch_batch_direct(int L, bool mat, const void* qs, uint32_t nq,
                                uint64_t unit_begin, uint64_t total, unsigned long long* rows,
                                unsigned long long* counts, unsigned long long* csums,
-                               hipStream_t s) {
+                               const uint64_t* order, hipStream_t s) {
   if (total == 0) return hipSuccess;
   const BatchQ* q = (const BatchQ*)qs;
   switch (L) {
@@ -178,10 +203,11 @@ hipError_t launch_batch_direct(int L, bool mat, const void* qs, uint32_t nq,
   case LV:                                                                                     \
     if (mat)                                                                                   \
       hipLaunchKernelGGL((batch_direct_kernel<LV, true>), dim3(grid_units(total)), dim3(kBlock), \
-                         0, s, q, nq, unit_begin, total, rows, counts, csums);                 \
+                         0, s, q, nq, unit_begin, total, rows, counts, csums, order);          \
     else                                                                                       \
       hipLaunchKernelGGL((batch_direct_kernel<LV, false>), dim3(grid_units(total)),           \
-                         dim3(kBlock), 0, s, q, nq, unit_begin, total, rows, counts, csums);   \
+                         dim3(kBlock), 0, s, q, nq, unit_begin, total, rows, counts, csums,    \
+                         order);                                                               \
     break;
     C(1) C(2) C(3) C(4) C(5)
 #undef C

@@ -19,14 +19,19 @@ hipError_t launch_batch_totals(const void* qs, uint32_t nq, const uint64_t* base
 size_t batch_desc_bytes();
 void batch_fill_desc(void* dst, const Plan& P, const ProbeOut& out, uint64_t unit0,
                      uint64_t units, uint64_t stride);
+hipError_t launch_batch_weights(const void* qs, uint32_t nq, uint64_t unit_begin,
+                                uint64_t total, uint64_t* keys, hipStream_t s);
 hipError_t launch_batch_direct(int L, bool mat, const void* qs, uint32_t nq,
                                uint64_t unit_begin, uint64_t total, unsigned long long* rows,
                                unsigned long long* counts, unsigned long long* csums,
-                               hipStream_t s);
+                               const uint64_t* order, hipStream_t s);
 
 }  // namespace amj
 
 using namespace amj;
+
+// Direct-output launches with at least this many units take them heaviest first.
+constexpr uint64_t kBatchOrderMinUnits = 2048;
 
 struct amj_batch {
   amj_ctx_t* ctx = nullptr;
@@ -382,8 +387,21 @@ int run_direct(amj_batch_t* b, const std::vector<uint32_t>& idx,
     const bool mat = b->qs[order[k]].materialize;
     while (e < nq && b->qs[order[e]].plan.n_levels == L && b->qs[order[e]].materialize == mat) ++e;
     const uint64_t gu = unit0[e - 1] + units[e - 1] - unit0[k];
+    // heaviest units first when they outnumber the device's wave slots (the sort buffers are
+    // sized for the whole batch, so no launch in flight loses its order to a reallocation)
+    const uint64_t* uorder = nullptr;
+    if (gu >= kBatchOrderMinUnits) {
+      void *keys = nullptr, *alt = nullptr, *tmp = nullptr;
+      AMJ_TRY(scratch(ctx, SLOT_SORT_KEYS, total * 8, &keys));
+      AMJ_TRY(scratch(ctx, SLOT_SORT_ALT, total * 8, &alt));
+      AMJ_TRY(scratch(ctx, SLOT_SORT_TEMP, radix_sort_temp_bytes(total), &tmp));
+      AMJ_HIP_TRY(ctx, launch_batch_weights((const uint8_t*)ddesc + db * k, e - k, unit0[k], gu,
+                                            (uint64_t*)keys, s));
+      AMJ_HIP_TRY(ctx, radix_sort_u64((uint64_t*)keys, (uint64_t*)alt, gu, 0xf0u, tmp, s));
+      uorder = (const uint64_t*)keys;
+    }
     AMJ_HIP_TRY(ctx, launch_batch_direct(L, mat, (const uint8_t*)ddesc + db * k, e - k, unit0[k],
-                                         gu, rows + k, counts + k, csums + k, s));
+                                         gu, rows + k, counts + k, csums + k, uorder, s));
     k = e;
   }
   std::vector<uint64_t> h(3 * (size_t)nq);
