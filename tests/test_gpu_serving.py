@@ -226,3 +226,32 @@ def test_server_request_larger_than_budget_fails_cleanly():
         # count-only requests hold no result memory and still run
         cnt, cs, _lat, _ = srv.wait(srv.submit(ids, materialize=False))
         assert (cnt, cs) == O.mhj(ch)[:2]
+
+
+def test_direct_output_batch_heavy_first_order(ctx):
+    # launches of at least 2,048 units take them heaviest first; every query's result is still
+    # the one it has alone (count, checksum and the order-free fingerprint of its tuples)
+    chains = [G.chain_relations([150_000, 40_000, 40_000], 20_000, seed=s, zipf_s=z)
+              for s, z in ((11, 1.0), (12, 0.5))]
+    R = [[ctx.relation(c, ids) for c, ids in ch] for ch in chains]
+    H = [[ctx.hash_table(r) for r in rs[1:]] for rs in R]
+    src, dst = G.rmat_oriented(17, 16, seed=5)
+    E = ctx.relation([src, dst], [0, 1])
+    HE = ctx.hash_table(E, 0, sort_vals=True)
+    assert sum(-(-len(c[0][0][0]) // 64) for c in chains) >= 2048 and len(src) // 64 >= 2048
+    want = [ctx.execute(R[k][0], H[k]) for k in range(2)]
+    want.append(ctx.execute_pattern(E, HE, "TRI"))
+    fps = [ctx.md5(w) for w in want]
+    b = Batch(ctx)
+    ids = [b.add(R[k][0], H[k], dro=1) for k in range(2)]
+    ids.append(b.add_pattern(E, HE, "TRI", dro=1))
+    b.run()
+    for qid, w, fp in zip(ids, want, fps):
+        r = b.result(qid)
+        assert (r.count, r.checksum) == (w.count, w.checksum)
+        assert r.rows_written == w.count
+        assert ctx.md5(r) == fp
+        ctx.free(r)
+    b.close()
+    for w in want:
+        ctx.free(w)
