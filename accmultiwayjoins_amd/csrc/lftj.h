@@ -68,9 +68,11 @@ hipError_t launch_lftj_write(const LPlan& P, const uint64_t* unit_base, const LO
                              bool hash, uint64_t* block_sums, uint64_t* n_blocks,
                              const uint64_t* order, hipStream_t s,
                              uint64_t* block_counts = nullptr);
-// Counts of `units` units k * stride + offset (an evenly strided sample when stride > 1).
-hipError_t launch_lftj_count(const LPlan& P, uint64_t* counts, hipStream_t s, uint64_t units,
-                             uint64_t stride, uint64_t offset);
+// Output-size sample: counts[k] = results of the roots (k * per_wave + i) * stride + offset,
+// i < per_wave (per_wave <= 64), for k < waves.
+hipError_t launch_lftj_sample(const LPlan& P, uint64_t* counts, uint64_t waves,
+                              uint32_t per_wave, uint64_t stride, uint64_t offset,
+                              hipStream_t s);
 // Direct output: one pass, rows reserved with atomics into *rows (zeroed by the caller); rows at
 // or past `capacity` are dropped, so *rows > capacity means the pass must run again.
 hipError_t launch_lftj_direct(const LPlan& P, const LOut& cols, uint64_t capacity, bool hash,

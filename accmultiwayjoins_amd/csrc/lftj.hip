@@ -20,11 +20,9 @@
 
 namespace amj {
 
-// Level 0: 64 consecutive level-0 values of driver0.
+// Level 0 from root r of driver0 on every lane where `valid`.
 template <int A>
-__device__ __forceinline__ void level0(const LPlan& P, uint64_t unit, LLv<A>& l) {
-  const uint64_t r = unit * kWave + (uint64_t)lane_id();
-  const bool valid = r < P.roots;
+__device__ __forceinline__ void level0_at(const LPlan& P, uint64_t r, bool valid, LLv<A>& l) {
   sfor<0, A>([&](auto a) { l.b[a] = 0; });
   sfor<0, kR>([&](auto q) { l.lo[q] = 0u; l.hi[q] = 0u; });
   int32_t v = 0;
@@ -35,6 +33,13 @@ __device__ __forceinline__ void level0(const LPlan& P, uint64_t unit, LLv<A>& l)
   l.b[0] = v;
   choose_driver(P, 1, ok, l);
   finish(l);
+}
+
+// Level 0 of a unit: 64 consecutive level-0 values of driver0.
+template <int A>
+__device__ __forceinline__ void level0(const LPlan& P, uint64_t unit, LLv<A>& l) {
+  const uint64_t r = unit * kWave + (uint64_t)lane_id();
+  level0_at(P, r, r < P.roots, l);
 }
 
 // What happens to the candidates that pass the last attribute (`ok` per lane, bindings in b):
@@ -173,26 +178,42 @@ __global__ __launch_bounds__(kBlock) void lftj_unit_weight_kernel(const LPlan P,
   if (lane_id() == 0) keys[k] = weight_key(w, k);
 }
 
-// Count pass over units k * stride + offset (stride > 1: an evenly strided sample), counts[k];
-// or, with an order, over every unit in that order, counts[unit].
+// Count pass: counts[unit] for every unit, taken in `order` (nullptr: number order).
 template <int A>
 __global__ __launch_bounds__(kBlock) void lftj_count_kernel(const LPlan P,
                                                             uint64_t* __restrict__ counts,
-                                                            uint64_t units, uint64_t stride,
-                                                            uint64_t offset,
+                                                            uint64_t units,
                                                             const uint64_t* __restrict__ order) {
   const uint64_t k = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
   if (k >= units) return;  // no barriers in this kernel
-  const uint64_t unit = order ? unit_of(order, k) : k * stride + offset;
+  const uint64_t unit = unit_of(order, k);
   LLv<A> l0;
   level0(P, unit, l0);
   CountSink sink;
   if (l0.T) descend<A, 1>(P, l0, sink);
-  if (lane_id() == 0) counts[order ? unit : k] = sink.count;
+  if (lane_id() == 0) counts[unit] = sink.count;
 }
 
-// Write pass at the scanned bases; or, with base == nullptr (nothing materialised), the only
-// pass: it hashes every result and leaves per-block result counts in block_counts.
+// Output-size sample: wave k counts the results of the roots (k * per_wave + i) * stride +
+// offset, i < per_wave. A few roots per wave instead of a unit of 64: the launch lasts as long
+// as its slowest wave, and on a skewed graph a unit of 64 roots costs several times its
+// heaviest root.
+template <int A>
+__global__ __launch_bounds__(kBlock) void lftj_sample_kernel(const LPlan P,
+                                                             uint64_t* __restrict__ counts,
+                                                             uint64_t waves, uint32_t per_wave,
+                                                             uint64_t stride, uint64_t offset) {
+  const uint64_t k = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+  if (k >= waves) return;  // no barriers in this kernel
+  const uint32_t lane = (uint32_t)lane_id();
+  const uint64_t r = (k * per_wave + lane) * stride + offset;
+  LLv<A> l0;
+  level0_at(P, r, lane < per_wave && r < P.roots, l0);
+  CountSink sink;
+  if (l0.T) descend<A, 1>(P, l0, sink);
+  if (lane == 0) counts[k] = sink.count;
+}
+
 template <int A, int MODE>
 __global__ __launch_bounds__(kBlock) void lftj_write_kernel(const LPlan P,
                                                             const uint64_t* __restrict__ base,
@@ -241,12 +262,11 @@ __global__ __launch_bounds__(kBlock) void lftj_direct_kernel(const LPlan P, cons
 uint64_t lftj_units(const LPlan& P) { return (P.roots + kWave - 1) / kWave; }
 
 static hipError_t launch_count(const LPlan& P, uint64_t* counts, hipStream_t s, uint64_t units,
-                               uint64_t stride, uint64_t offset, const uint64_t* order) {
+                               const uint64_t* order) {
   if (units == 0) return hipSuccess;
   const dim3 grid((unsigned)((units + kWavesPerBlock - 1) / kWavesPerBlock));
 #define LC(AV)                                                                                \
-  hipLaunchKernelGGL(lftj_count_kernel<AV>, grid, dim3(kBlock), 0, s, P, counts, units, stride, \
-                     offset, order)
+  hipLaunchKernelGGL(lftj_count_kernel<AV>, grid, dim3(kBlock), 0, s, P, counts, units, order)
   switch (P.n_attrs) {
     case 2: LC(2); break;
     case 3: LC(3); break;
@@ -257,14 +277,27 @@ static hipError_t launch_count(const LPlan& P, uint64_t* counts, hipStream_t s, 
   return hipGetLastError();
 }
 
-hipError_t launch_lftj_count(const LPlan& P, uint64_t* counts, hipStream_t s, uint64_t units,
-                             uint64_t stride, uint64_t offset) {
-  return launch_count(P, counts, s, units, stride, offset, nullptr);
+hipError_t launch_lftj_sample(const LPlan& P, uint64_t* counts, uint64_t waves,
+                              uint32_t per_wave, uint64_t stride, uint64_t offset,
+                              hipStream_t s) {
+  if (waves == 0) return hipSuccess;
+  const dim3 grid((unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock));
+#define LS(AV)                                                                                 \
+  hipLaunchKernelGGL(lftj_sample_kernel<AV>, grid, dim3(kBlock), 0, s, P, counts, waves, per_wave, \
+                     stride, offset)
+  switch (P.n_attrs) {
+    case 2: LS(2); break;
+    case 3: LS(3); break;
+    case 4: LS(4); break;
+    default: return hipErrorInvalidValue;
+  }
+#undef LS
+  return hipGetLastError();
 }
 
 hipError_t launch_lftj_count(const LPlan& P, uint64_t* counts, const uint64_t* order,
                              hipStream_t s) {
-  return launch_count(P, counts, s, lftj_units(P), 1, 0, order);
+  return launch_count(P, counts, s, lftj_units(P), order);
 }
 
 hipError_t launch_lftj_unit_weights(const LPlan& P, uint64_t* keys, hipStream_t s) {

@@ -91,11 +91,12 @@ static int make_lplan(amj_ctx_t* ctx, const std::vector<LUse>& uses, const uint3
   return AMJ_OK;
 }
 
-constexpr uint64_t kLftjSampleUnits = 1024;
+constexpr uint64_t kLftjSampleRoots = 65536;
+constexpr uint32_t kLftjSampleRootsPerWave = 4;
 
 // DFS with direct output (opts.dro, materialising): the output is sized from the counts of an
-// evenly strided sample of at most kLftjSampleUnits units (x1.5 + 64 Ki rows; exact when every unit is
-// sampled, or opts.dro_capacity_rows), one pass writes through per-wave LDS buffers with one
+// evenly strided sample of at most kLftjSampleRoots roots (x1.5 + 64 Ki rows; exact when every
+// root is sampled, or opts.dro_capacity_rows), one pass writes through per-wave LDS buffers with one
 // atomic reservation per 256 rows, and a pass that overran its output runs once more into an
 // exact one. Count and checksum are those of the count/write engine; the row order is not.
 // Launch order of the DFS units, heaviest first, when there are more units than wave slots
@@ -127,20 +128,26 @@ static int run_lftj_direct(amj_ctx_t* ctx, const LPlan& P, const amj_opts_t* o,
   uint64_t capacity = o->dro_capacity_rows;
   uint32_t passes = 0;
   if (capacity == 0 && units > 0) {
-    // the sample's units sit mid-stride: unit 0 holds the smallest roots, on skewed graphs the
-    // heaviest ones, and one heavy unit would set the sample launch's length by itself
-    const uint64_t su = std::min<uint64_t>(units, kLftjSampleUnits), stride = units / su;
+    // a strided sample of kLftjSampleRoots roots, kLftjSampleRootsPerWave per wave: the launch
+    // lasts as long as its slowest wave, and a wave of few roots costs about its heaviest root
+    // (a unit of 64 contiguous roots costs several times that on a skewed graph)
+    const uint64_t roots = P.roots;
+    const uint64_t sr = std::min<uint64_t>(roots, kLftjSampleRoots);
+    const uint64_t stride = roots / sr;
+    const uint64_t waves = (sr + kLftjSampleRootsPerWave - 1) / kLftjSampleRootsPerWave;
     void *counts = nullptr, *tmp = nullptr, *sum = nullptr;
-    AMJ_TRY(scratch(ctx, SLOT_UNIT_COUNTS, (su + 1) * 8, &counts));
-    AMJ_TRY(scratch(ctx, SLOT_SCAN_TEMP, scan_temp_elems(su + 1) * 8, &tmp));
+    AMJ_TRY(scratch(ctx, SLOT_UNIT_COUNTS, (waves + 1) * 8, &counts));
+    AMJ_TRY(scratch(ctx, SLOT_SCAN_TEMP, scan_temp_elems(waves + 1) * 8, &tmp));
     AMJ_TRY(scratch(ctx, SLOT_SUM, 64, &sum));
-    AMJ_HIP_TRY(ctx, launch_lftj_count(P, (uint64_t*)counts, s, su, stride, stride / 2));
-    AMJ_HIP_TRY(ctx, reduce_sum_u64((const uint64_t*)counts, su, (uint64_t*)sum, (uint64_t*)tmp, s));
+    AMJ_HIP_TRY(ctx, launch_lftj_sample(P, (uint64_t*)counts, waves, kLftjSampleRootsPerWave,
+                                        stride, stride / 2, s));
+    AMJ_HIP_TRY(ctx, reduce_sum_u64((const uint64_t*)counts, waves, (uint64_t*)sum,
+                                    (uint64_t*)tmp, s));
     uint64_t sampled = 0;
     AMJ_HIP_TRY(ctx, hipMemcpyAsync(&sampled, sum, 8, hipMemcpyDeviceToHost, s));
     AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
-    capacity = su == units ? sampled
-                           : (uint64_t)((double)sampled * (double)units / (double)su * 1.5) + 65536;
+    capacity = sr == roots ? sampled
+                           : (uint64_t)((double)sampled * (double)roots / (double)sr * 1.5) + 65536;
     ++passes;
   }
   const uint64_t* order = nullptr;
