@@ -18,7 +18,7 @@ struct BatchQ {
   ProbeOut out;     // capacity 0: count and checksum only
   uint64_t unit0;   // first unit of this query in the batch
   uint64_t units;
-  uint64_t stride;  // unit k of the query covers probe rows from k * stride * 64 (sampling)
+  uint64_t stride;  // > 1: a sample, wave k takes the rows (k * 64 + lane) * stride + stride / 2
 };
 
 __device__ __forceinline__ uint32_t query_of(const BatchQ* __restrict__ qs, uint32_t nq,
@@ -43,7 +43,14 @@ __global__ __launch_bounds__(kBlock) void batch_count_kernel(const BatchQ* __res
   const uint32_t q = uniform(query_of(qs, nq, g));
   const BatchQ& Q = qs[q];
   Lv l0;
-  make_level0(Q.P, (g - Q.unit0) * Q.stride * kWave, l0);
+  if (Q.stride > 1) {
+    // a sample: wave i of the query takes the rows (i * 64 + lane) * stride + stride / 2, so
+    // one wave never holds a run of neighbouring rows (edges sorted by source share one hub)
+    const uint64_t r = ((g - Q.unit0) * kWave + (uint64_t)lane_id()) * Q.stride + Q.stride / 2;
+    make_level0_at(Q.P, r, r < Q.P.probe_n, l0);
+  } else {
+    make_level0(Q.P, (g - Q.unit0) * kWave, l0);
+  }
   Acc acc{};
   if (l0.T) descend<L, 0, MODE_COUNT>(Q.P, l0, ProbeOut{}, acc);
   if (lane_id() == 0) counts[g] = acc.count;

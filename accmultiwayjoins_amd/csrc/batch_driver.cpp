@@ -204,7 +204,7 @@ std::vector<uint32_t> depth_order(amj_batch_t* b, std::vector<uint32_t> idx) {
   return idx;
 }
 
-// Count pass over every unit (or over a strided sample of units: stride[k] > 1), one scan;
+// Count pass over every unit (or over a strided sample of rows: stride[k] > 1), one scan;
 // returns per-query totals. `desc` receives the descriptors (count-only outputs).
 int batch_counts(amj_batch_t* b, const std::vector<uint32_t>& order,
                         const std::vector<uint64_t>& units, const std::vector<uint64_t>& stride,

@@ -50,14 +50,19 @@ __device__ __forceinline__ void lookup_level(const Plan& P, int t, Lv& l, bool v
   lookup_run(P.tab[t], key, check, cval, valid, l.lo, l.len);
 }
 
-__device__ __forceinline__ void make_level0(const Plan& P, uint64_t row0, Lv& l) {
-  const uint64_t r = row0 + (uint64_t)lane_id();
-  const bool valid = r < P.probe_n;
+// Level 0 from probe row r on every lane where `valid`.
+__device__ __forceinline__ void make_level0_at(const Plan& P, uint64_t r, bool valid, Lv& l) {
   sfor<0, kMaxAttrs>([&](auto a) { l.b[a] = 0; });
   for (int c = 0; c < P.probe_ncols; ++c)
     set_attr(l.b, P.probe_attr[c], valid ? P.probe_cols[c][r] : 0);
   lookup_level(P, 0, l, valid);
   finish_level(l);
+}
+
+// Level 0 of probe rows [row0, row0 + 64).
+__device__ __forceinline__ void make_level0(const Plan& P, uint64_t row0, Lv& l) {
+  const uint64_t r = row0 + (uint64_t)lane_id();
+  make_level0_at(P, r, r < P.probe_n, l);
 }
 
 // Children [c, min(c+64, end)) of level t become the level-(t+1) frontier (end <= par.T).
