@@ -115,7 +115,8 @@ typedef struct {
   uint32_t passes;              /* probe traversals run (dro: 1, or 2 after an overflow) */
 } amj_result_t;
 
-#define AMJ_PATH_PLAIN 1        /* count pass + scan + write pass */
+#define AMJ_PATH_PLAIN 1        /* count pass + scan + write pass (materialize = 0: one
+                                   pass that counts and hashes) */
 #define AMJ_PATH_FIB 2          /* subtree-count buffer + output-window writer */
 #define AMJ_PATH_WS 3
 #define AMJ_PATH_DRO 4
@@ -196,8 +197,9 @@ void amj_autotune_forget(amj_ctx_t* ctx);
  * equals the one amj_execute gives for the same query (count, checksum, tuples in the same
  * order). Queries added with opts->dro = 1 skip the count pass instead (the direct result
  * output of AMHJ_DRO, Joins/AMHJ_DRO.cuh:18-707, applied to the whole batch): each output is
- * sized from a strided sample of the query's probe units (or from opts->dro_capacity_rows),
- * one launch reserves rows with atomics, and a query whose results overrun its size is run
+ * sized from a strided sample of the query's probe rows (or from opts->dro_capacity_rows),
+ * one launch per depth reserves rows with one atomic per 256 staged rows, taking its units
+ * heaviest first, and a query whose results overrun its size is run
  * again exactly; counts and checksums are the same, the row order is not. dro queries are
  * admitted in the order they were added while their outputs fit the budget: the rest is not
  * run and amj_batch_result returns AMJ_ERR_NOMEM for it. Results are taken with
