@@ -53,7 +53,8 @@ class AmjOpts(C.Structure):
         ("fib_suffix", C.c_int),
         ("probe_order", C.c_int),
         ("engine", C.c_uint32),
-        ("reserved", C.c_uint32 * 4),
+        ("ws_stall_ms", C.c_uint32),
+        ("reserved", C.c_uint32 * 3),
     ]
 
 

@@ -24,69 +24,23 @@
 // allocation.
 #include "probe_engine.h"
 #include "runtime.h"
+#include "ws_queue.h"
 
 namespace amj {
 
-// Control block of one launch (zeroed by the host).
-struct DynCtl {
-  unsigned long long next_unit;  // units claimed
-  unsigned long long q_tail;     // task slots reserved
-  unsigned long long q_head;     // task slots claimed
-  unsigned long long out_rows;   // output rows reserved (== results once the pass ends)
-  unsigned long long count;      // results
-  unsigned long long csum;       // checksum, sum of the waves' partials mod 2^64
-  unsigned long long pushed;     // tasks published
-  unsigned long long progress;   // units and tasks completed
-  unsigned int busy;             // waves holding a unit or a task
-  unsigned int flags;            // 1: a wait hit the time limit
-};
-
-constexpr uint64_t kMaxClaim = 64;  // units claimed at once by a wave
-constexpr uint32_t kFlagSpin = 1u;
+using DynCtl = WsCtl;
+using DynArgs = WsArgs;
 
 // Per-wave state of the persistent engine besides the accumulators.
 struct DynWave {
   DirectStage st;  // staged output rows (materialising passes)
-  bool qfull;      // a task reservation failed: the queue has no room left in this launch
+  WsWave q;        // sharing state
 };
 
 // Task record: 16 header words then 10 words per lane (bindings, lo, len, excl).
 constexpr int kTaskHdr = 16;
 constexpr int kTaskLaneWords = kMaxAttrs + 4;
 constexpr int kTaskWords = kTaskHdr + kTaskLaneWords * kWave;
-
-struct DynArgs {
-  DynCtl* ctl;
-  uint32_t* tasks;   // cap * kTaskWords
-  uint32_t* ready;   // cap flags
-  uint64_t cap;      // task slots
-  uint64_t split;    // children per task (multiple of 64); 0 disables sharing
-  uint64_t units;    // units of 64 probe rows
-  uint64_t out_cap;  // output rows available
-  uint64_t stall_ticks;  // wall-clock ticks without any completed work before a wave gives up
-};
-
-// Every access to the control block is relaxed: an acquire at agent scope invalidates the
-// CU's vector caches, and idle waves polling with acquires (or workers finishing chunks with
-// acquire-release updates) evicted the table runs the working waves on the same CU were
-// reading — ten times slower on FOUR. No data is published through these counters: a task
-// record is published by its own ready flag (release store, acquire by the taker), and a wave
-// that leaves early on a stale view only gives up parallelism (a pusher serves its own tasks).
-__device__ __forceinline__ unsigned long long ld_rlx(const unsigned long long* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ unsigned int ld_rlx(const unsigned int* p) {
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-template <typename T>
-__device__ __forceinline__ T add_relaxed(T* p, T v) {
-  return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Wave-uniform result of an atomic done by lane 0.
-__device__ __forceinline__ uint64_t lane0_u64(uint64_t v) {
-  return uniform(__shfl(v, 0, kWave));
-}
 
 // Publish the level frontier `l` with child range [c0, c1) into slot `slot`.
 template <int t>
@@ -109,8 +63,7 @@ __device__ __forceinline__ void write_task(const DynArgs& A, uint64_t slot, cons
     rec[5] = (uint32_t)l.T;
     rec[6] = (uint32_t)(l.T >> 32);
   }
-  __threadfence();  // the record is visible device-wide before its flag
-  if (lane == 0) __hip_atomic_store(&A.ready[slot], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  publish_task(A, slot);
 }
 
 __device__ __forceinline__ void read_task(const DynArgs& A, uint64_t slot, int& t, uint64_t& c0,
@@ -128,60 +81,39 @@ __device__ __forceinline__ void read_task(const DynArgs& A, uint64_t slot, int& 
   l.excl = ((uint64_t)lw[(kMaxAttrs + 3) * kWave + lane] << 32) | lw[(kMaxAttrs + 2) * kWave + lane];
 }
 
-// Reserve up to k task slots; returns the first slot, the number granted in *got.
-__device__ __forceinline__ uint64_t reserve_tasks(const DynArgs& A, uint64_t k, uint64_t* got) {
-  uint64_t first = 0, n = 0;
-  if (lane_id() == 0) {
-    for (int attempt = 0; attempt < 64; ++attempt) {
-      unsigned long long tl = ld_rlx(&A.ctl->q_tail);
-      if (tl >= A.cap) break;
-      const unsigned long long room = A.cap - tl;
-      const unsigned long long want = k < room ? k : room;
-      if (__hip_atomic_compare_exchange_strong(&A.ctl->q_tail, &tl, tl + want, __ATOMIC_RELAXED,
-                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
-        first = tl;
-        n = want;
-        break;
-      }
-    }
-  }
-  *got = lane0_u64(n);
-  return lane0_u64(first);
-}
-
 template <int L, int t, int MODE>
 __device__ __forceinline__ void dyn_descend(const Plan& P, const Lv& cur, uint64_t c0, uint64_t c1,
                             const ProbeOut& out, const DynArgs& A, Acc& acc, DynWave& W) {
+  const auto step = [&]() { ws_step(A, W.q); };
   if constexpr (t == L - 1) {
     // the last level is never split: its children are the results, [0, T)
     if constexpr (MODE == MODE_COUNT) {
       acc.count += cur.T;
     } else if constexpr (!mode_mat<MODE>()) {
-      emit<false, mode_hash<MODE>()>(P, t, cur, 0, out, acc);
+      emit<false, mode_hash<MODE>()>(P, t, cur, 0, out, acc, step);
       acc.count += cur.T;
     } else {
-      direct_emit<true, mode_hash<MODE>()>(P, t, cur, out, acc, W.st, &A.ctl->out_rows);
+      direct_emit<true, mode_hash<MODE>()>(P, t, cur, out, acc, W.st, &A.ctl->out_rows, step);
       acc.count += cur.T;
     }
   } else {
     for (uint64_t c = c0; c < c1; c += kWave) {
-      if (A.split && !W.qfull && c1 - c > A.split + (uint64_t)kWave) {
+      step();
+      if (!W.q.qfull && ws_should_split(c, c1, A.split)) {
         // keep [c, c + split), publish the remaining pieces (the last ones if the queue is short)
-        const uint64_t keep_end = c + A.split;
-        const uint64_t k = (c1 - keep_end + A.split - 1) / A.split;
+        const uint64_t k = ws_pieces(c, c1, A.split);
         uint64_t got = 0;
-        const uint64_t slot = reserve_tasks(A, k, &got);
+        bool full = false;
+        const uint64_t slot = reserve_tasks(A, k, &got, &full);
+        if (full) W.q.qfull = true;  // slots are never reused within a launch: stop asking
         if (got) {
-          const uint64_t first_piece = k - got;
           for (uint64_t j = 0; j < got; ++j) {
-            const uint64_t p0 = keep_end + (first_piece + j) * A.split;
-            const uint64_t p1 = p0 + A.split < c1 ? p0 + A.split : c1;
+            uint64_t p0 = 0, p1 = 0;
+            ws_piece(c, c1, A.split, k, got, j, &p0, &p1);
             write_task<t>(A, slot + j, cur, p0, p1);
           }
           if (lane_id() == 0) add_relaxed(&A.ctl->pushed, (unsigned long long)got);
-          c1 = keep_end + first_piece * A.split;
-        } else {
-          W.qfull = true;  // slots are never reused within a launch: stop asking
+          c1 = ws_keep_end(c, c1, A.split, k, got);
         }
       }
       Lv ch;
@@ -191,118 +123,34 @@ __device__ __forceinline__ void dyn_descend(const Plan& P, const Lv& cur, uint64
   }
 }
 
-// Claim a published task; -1: none available, -2: its record never became ready.
-__device__ __forceinline__ int64_t try_pop(const DynArgs& A) {
-  int64_t r = -1;
-  if (lane_id() == 0) {
-    unsigned long long h = ld_rlx(&A.ctl->q_head);
-    const unsigned long long tl = ld_rlx(&A.ctl->q_tail);
-    if (h < tl && __hip_atomic_compare_exchange_strong(&A.ctl->q_head, &h, h + 1,
-                                                       __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                       __HIP_MEMORY_SCOPE_AGENT)) {
-      r = (int64_t)h;
-      const uint64_t t0 = wall_clock64();
-      while (__hip_atomic_load(&A.ready[h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
-        if (wall_clock64() - t0 > A.stall_ticks) {
-          __hip_atomic_fetch_or(&A.ctl->flags, kFlagSpin, __ATOMIC_RELAXED,
-                                __HIP_MEMORY_SCOPE_AGENT);
-          r = -2;
-          break;
-        }
-        __builtin_amdgcn_s_sleep(2);
-      }
-    }
-  }
-  r = (int64_t)lane0_u64((uint64_t)r);
-  if (r >= 0)  // every lane acquires the record (its flag is set by now)
-    (void)__hip_atomic_load(&A.ready[r], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-  return r;
-}
-
 template <int L, int MODE>
 __global__ __launch_bounds__(kBlock) void dyn_kernel(const Plan P, const ProbeOut out,
                                                      const DynArgs A) {
   constexpr bool MAT = mode_mat<MODE>();
   __shared__ int32_t stage_mem[MAT ? kWavesPerBlock * kMaxAttrs * kDirectRows : 1];
   DynWave W{{(lds_i32*)&stage_mem[MAT ? (threadIdx.x / kWave) * kMaxAttrs * kDirectRows : 0], 0u},
-            false};
+            {false, 0u}};
   Acc acc{};
-  uint64_t seen = ~0ull, since = 0;  // progress counter last seen, and when it last moved
-  for (;;) {
-    // busy counts waves holding claimed work; it only keeps idle waves around to help (a wave
-    // that leaves early costs parallelism, never work: a pusher always serves its own tasks)
-    // Units are claimed in guided chunks (a quarter of the remaining units per wave, 1 to 64):
-    // four counter updates per claim instead of per unit, since one cheap unit costs less than
-    // the atomics on these shared counters.
-    uint64_t u = ~0ull, nu = 0;
-    if (lane_id() == 0) {
-      const unsigned long long seen_u = ld_rlx(&A.ctl->next_unit);
-      if (seen_u < A.units) {
-        const uint64_t waves = (uint64_t)gridDim.x * kWavesPerBlock;
-        uint64_t k = (A.units - seen_u) / (4 * waves);
-        k = k < 1 ? 1 : (k > kMaxClaim ? kMaxClaim : k);
-        u = add_relaxed(&A.ctl->next_unit, (unsigned long long)k);
-        nu = k;
-      }
-    }
-    u = lane0_u64(u);
-    nu = lane0_u64(nu);
-    if (u < A.units) {
-      const uint64_t ue = u + nu < A.units ? u + nu : A.units;
-      if (lane_id() == 0) add_relaxed(&A.ctl->busy, 1u);
-      for (uint64_t v = u; v < ue; ++v) {
-        Lv l0;
-        make_level0(P, v * kWave, l0);
-        if (l0.T) dyn_descend<L, 0, MODE>(P, l0, 0, l0.T, out, A, acc, W);
-      }
-      if (lane_id() == 0) {
-        add_relaxed(&A.ctl->progress, 1ull);
-        add_relaxed(&A.ctl->busy, ~0u);
-      }
-      continue;
-    }
-    if (A.split == 0) break;  // without sharing there are no tasks to wait for
-    const int64_t slot = try_pop(A);
-    if (slot >= 0) {
-      if (lane_id() == 0) add_relaxed(&A.ctl->busy, 1u);
-      int t = 0;
-      uint64_t c0 = 0, c1 = 0;
-      Lv l;
-      read_task(A, (uint64_t)slot, t, c0, c1, l);
-      sfor<0, L - 1>([&](auto tt) {
-        if ((int)tt == t) dyn_descend<L, tt, MODE>(P, l, c0, c1, out, A, acc, W);
+  ws_serve(
+      A,
+      [&](uint64_t u, uint64_t ue) {
+        for (uint64_t v = u; v < ue; ++v) {
+          Lv l0;
+          make_level0(P, v * kWave, l0);
+          if (l0.T) dyn_descend<L, 0, MODE>(P, l0, 0, l0.T, out, A, acc, W);
+        }
+      },
+      [&](uint64_t slot) {
+        int t = 0;
+        uint64_t c0 = 0, c1 = 0;
+        Lv l;
+        read_task(A, slot, t, c0, c1, l);
+        sfor<0, L - 1>([&](auto tt) {
+          if ((int)tt == t) dyn_descend<L, tt, MODE>(P, l, c0, c1, out, A, acc, W);
+        });
       });
-      if (lane_id() == 0) {
-        add_relaxed(&A.ctl->progress, 1ull);
-        add_relaxed(&A.ctl->busy, ~0u);
-      }
-      continue;
-    }
-    if (slot == -2) break;
-    uint32_t done = 0;
-    if (lane_id() == 0)
-      done = (ld_rlx(&A.ctl->busy) == 0u && ld_rlx(&A.ctl->q_head) >= ld_rlx(&A.ctl->q_tail))
-                 ? 1u : 0u;
-    if (uniform(__shfl(done, 0, kWave))) break;
-    // no work here while other waves hold some: give up only if nothing completes for long
-    const uint64_t prog = lane0_u64(lane_id() == 0 ? ld_rlx(&A.ctl->progress) : 0ull);
-    const uint64_t now = wall_clock64();
-    if (prog != seen) {
-      seen = prog;
-      since = now;
-    } else if (now - since > A.stall_ticks) {
-      if (lane_id() == 0)
-        __hip_atomic_fetch_or(&A.ctl->flags, kFlagSpin, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      break;
-    }
-    __builtin_amdgcn_s_sleep(64);
-  }
   if (MAT) direct_flush(out, P.n_attrs, W.st, &A.ctl->out_rows);
-  const uint64_t cs = wave_sum_u64(acc.csum);
-  if (lane_id() == 0) {
-    add_relaxed(&A.ctl->count, (unsigned long long)acc.count);
-    add_relaxed(&A.ctl->csum, (unsigned long long)cs);
-  }
+  ws_accumulate(A, acc.count, acc.csum);
 }
 
 // Results of `samples` units spread evenly over the probe relation (the output estimate).
@@ -317,7 +165,7 @@ __global__ __launch_bounds__(kBlock) void dyn_sample_kernel(const Plan P, uint64
   make_level0(P, u * kWave, l0);
   Acc acc{};
   const DynArgs none{};  // no sharing, no output
-  DynWave W{{nullptr, 0u}, true};
+  DynWave W{{nullptr, 0u}, {true, 0u}};
   if (l0.T) dyn_descend<L, 0, MODE_COUNT>(P, l0, 0, l0.T, ProbeOut{}, none, acc, W);
   if (lane_id() == 0) add_relaxed(sum, (unsigned long long)acc.count);
 }
@@ -423,7 +271,7 @@ int run_dynamic(amj_ctx_t* ctx, Plan& P, const amj_opts_t* o, amj_result_t* res)
   if (hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, ctx->device) !=
           hipSuccess || clock_khz <= 0)
     clock_khz = 100000;
-  A.stall_ticks = kStallMs * (uint64_t)clock_khz;
+  A.stall_ticks = (o->ws_stall_ms ? (uint64_t)o->ws_stall_ms : kStallMs) * (uint64_t)clock_khz;
   if (o->ws) {
     A.split = o->ws_split_threshold ? (uint64_t)(o->ws_split_threshold + 63) / 64 * 64
                                     : kDefaultSplit;
@@ -589,7 +437,7 @@ int run_ooc(amj_ctx_t* ctx, Plan& P, const amj_opts_t* o, amj_result_t* res) {
   if (hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, ctx->device) !=
           hipSuccess || clock_khz <= 0)
     clock_khz = 100000;
-  A.stall_ticks = kStallMs * (uint64_t)clock_khz;
+  A.stall_ticks = (o->ws_stall_ms ? (uint64_t)o->ws_stall_ms : kStallMs) * (uint64_t)clock_khz;
   if (o->ws) {
     A.split = o->ws_split_threshold ? (uint64_t)(o->ws_split_threshold + 63) / 64 * 64
                                     : kDefaultSplit;

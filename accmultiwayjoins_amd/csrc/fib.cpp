@@ -59,8 +59,11 @@ static int window_pass(amj_ctx_t* ctx, const Plan& Q, const uint64_t* off, uint6
   return AMJ_OK;
 }
 
-int run_fib(amj_ctx_t* ctx, Plan& P, const amj_opts_t* o, amj_result_t* res) {
+int run_fib(amj_ctx_t* ctx, const Plan& plan, const amj_opts_t* o, amj_result_t* res) {
   hipStream_t s = ctx->stream;
+  // a local copy: the probe columns are redirected to the grouped copy and the tables get their
+  // prefix arrays, and the caller's plan stays as it was for a fallback engine
+  Plan P = plan;
   const int L = P.n_levels;
   const int lastv = P.val_mode[L - 1] == VAL_BIND ? 1 : 0;
   AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[2], s));

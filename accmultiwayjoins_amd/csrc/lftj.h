@@ -79,6 +79,15 @@ hipError_t launch_lftj_direct(const LPlan& P, const LOut& cols, uint64_t capacit
                               unsigned long long* rows, uint64_t* block_sums, uint64_t* n_blocks,
                               const uint64_t* order, hipStream_t s);
 
+// Work sharing (lftj_ws.hip): persistent waves over the units (taken in `order`, nullptr: number
+// order) that split long candidate runs into tasks. `args` is a WsArgs (ws_queue.h) whose task
+// buffer holds cap records of lftj_ws_task_bytes(). mode: MODE_COUNT / MODE_HASH count (and
+// hash) into the control block; MODE_WRITE / MODE_WRITE_HASH write with direct output.
+uint64_t lftj_ws_task_bytes();
+int lftj_ws_blocks_per_cu(const LPlan& P, int mode);
+hipError_t launch_lftj_ws(const LPlan& P, const LOut& cols, int mode, const void* args,
+                          const uint64_t* order, unsigned grid, hipStream_t s);
+
 // Level-0 arrays of a relation readable from the host.
 struct HostView {
   const int32_t* d0;

@@ -9,7 +9,9 @@
 #include <cstring>
 
 #include "lftj.h"
+#include "probe_common.h"
 #include "runtime.h"
+#include "ws_queue.h"
 
 namespace amj {
 
@@ -120,6 +122,35 @@ static int unit_order(amj_ctx_t* ctx, const LPlan& P, const uint64_t** order) {
   return AMJ_OK;
 }
 
+// Output size of a direct-output pass from a strided sample of kLftjSampleRoots roots,
+// kLftjSampleRootsPerWave per wave: the launch lasts as long as its slowest wave, and a wave of
+// few roots costs about its heaviest root (a unit of 64 contiguous roots costs several times
+// that on a skewed graph). x1.5 + 64 Ki rows; exact when every root is sampled.
+static int sample_capacity(amj_ctx_t* ctx, const LPlan& P, uint64_t* capacity) {
+  hipStream_t s = ctx->stream;
+  const uint64_t roots = P.roots;
+  *capacity = 0;
+  if (roots == 0) return AMJ_OK;
+  const uint64_t sr = std::min<uint64_t>(roots, kLftjSampleRoots);
+  const uint64_t stride = roots / sr;
+  const uint64_t waves = (sr + kLftjSampleRootsPerWave - 1) / kLftjSampleRootsPerWave;
+  void *counts = nullptr, *tmp = nullptr, *sum = nullptr;
+  AMJ_TRY(scratch(ctx, SLOT_UNIT_COUNTS, (waves + 1) * 8, &counts));
+  AMJ_TRY(scratch(ctx, SLOT_SCAN_TEMP, scan_temp_elems(waves + 1) * 8, &tmp));
+  AMJ_TRY(scratch(ctx, SLOT_SUM, 64, &sum));
+  AMJ_HIP_TRY(ctx, launch_lftj_sample(P, (uint64_t*)counts, waves, kLftjSampleRootsPerWave,
+                                      stride, stride / 2, s));
+  AMJ_HIP_TRY(ctx, reduce_sum_u64((const uint64_t*)counts, waves, (uint64_t*)sum,
+                                  (uint64_t*)tmp, s));
+  uint64_t sampled = 0;
+  AMJ_HIP_TRY(ctx, hipMemcpyAsync(&sampled, sum, 8, hipMemcpyDeviceToHost, s));
+  AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
+  *capacity = sr == roots
+                  ? sampled
+                  : (uint64_t)((double)sampled * (double)roots / (double)sr * 1.5) + 65536;
+  return AMJ_OK;
+}
+
 static int run_lftj_direct(amj_ctx_t* ctx, const LPlan& P, const amj_opts_t* o,
                            amj_result_t* res) {
   hipStream_t s = ctx->stream;
@@ -128,26 +159,7 @@ static int run_lftj_direct(amj_ctx_t* ctx, const LPlan& P, const amj_opts_t* o,
   uint64_t capacity = o->dro_capacity_rows;
   uint32_t passes = 0;
   if (capacity == 0 && units > 0) {
-    // a strided sample of kLftjSampleRoots roots, kLftjSampleRootsPerWave per wave: the launch
-    // lasts as long as its slowest wave, and a wave of few roots costs about its heaviest root
-    // (a unit of 64 contiguous roots costs several times that on a skewed graph)
-    const uint64_t roots = P.roots;
-    const uint64_t sr = std::min<uint64_t>(roots, kLftjSampleRoots);
-    const uint64_t stride = roots / sr;
-    const uint64_t waves = (sr + kLftjSampleRootsPerWave - 1) / kLftjSampleRootsPerWave;
-    void *counts = nullptr, *tmp = nullptr, *sum = nullptr;
-    AMJ_TRY(scratch(ctx, SLOT_UNIT_COUNTS, (waves + 1) * 8, &counts));
-    AMJ_TRY(scratch(ctx, SLOT_SCAN_TEMP, scan_temp_elems(waves + 1) * 8, &tmp));
-    AMJ_TRY(scratch(ctx, SLOT_SUM, 64, &sum));
-    AMJ_HIP_TRY(ctx, launch_lftj_sample(P, (uint64_t*)counts, waves, kLftjSampleRootsPerWave,
-                                        stride, stride / 2, s));
-    AMJ_HIP_TRY(ctx, reduce_sum_u64((const uint64_t*)counts, waves, (uint64_t*)sum,
-                                    (uint64_t*)tmp, s));
-    uint64_t sampled = 0;
-    AMJ_HIP_TRY(ctx, hipMemcpyAsync(&sampled, sum, 8, hipMemcpyDeviceToHost, s));
-    AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
-    capacity = sr == roots ? sampled
-                           : (uint64_t)((double)sampled * (double)roots / (double)sr * 1.5) + 65536;
+    AMJ_TRY(sample_capacity(ctx, P, &capacity));
     ++passes;
   }
   const uint64_t* order = nullptr;
@@ -230,7 +242,128 @@ static int run_lftj_count(amj_ctx_t* ctx, const LPlan& P, amj_result_t* res) {
   return AMJ_OK;
 }
 
+// Work sharing (opts.ws, ALFTJ.cuh:464-499): persistent waves over the units, long candidate
+// runs split into tasks of `split` candidates (opts.ws_split_threshold, default the reference's
+// BUC_THRESHOLD of 1,024). Without output one pass counts and hashes. Materialising, the
+// output is sized by a counting pass (or, with dro, from the sample of the direct-output path)
+// and written in one pass with direct output; an overrun runs once more into an exact size.
+constexpr uint64_t kLftjSplit = 1024;        // ALFTJ.cuh:22 BUC_THRESHOLD
+constexpr uint64_t kLftjTaskSlots = 8192;    // task records of 5,440 bytes
+constexpr uint64_t kLftjStallMs = 20000;     // no progress for this long: give up
+
+static int lftj_ws_pass(amj_ctx_t* ctx, const LPlan& P, const LOut& out, int mode,
+                        const WsArgs& A, const uint64_t* order, unsigned grid, WsCtl* hc,
+                        double* kms) {
+  hipStream_t s = ctx->stream;
+  AMJ_HIP_TRY(ctx, hipMemsetAsync(A.ctl, 0, sizeof(WsCtl), s));
+  if (A.split) AMJ_HIP_TRY(ctx, hipMemsetAsync(A.ready, 0, A.cap * 4, s));
+  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
+  if (A.units) AMJ_HIP_TRY(ctx, launch_lftj_ws(P, out, mode, &A, order, grid, s));
+  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
+  AMJ_HIP_TRY(ctx, hipMemcpyAsync(hc, A.ctl, sizeof(WsCtl), hipMemcpyDeviceToHost, s));
+  AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
+  *kms += elapsed_ms(ctx->ev[3], ctx->ev[4]);
+  if (hc->flags & kFlagSpin) {
+    set_error(ctx, AMJ_ERR_STATE, "ALFTJ work sharing: no progress within the time limit");
+    return AMJ_ERR_STATE;
+  }
+  return AMJ_OK;
+}
+
+static int run_lftj_ws(amj_ctx_t* ctx, const LPlan& P, const amj_opts_t* o, amj_result_t* res) {
+  hipStream_t s = ctx->stream;
+  const uint64_t units = lftj_units(P);
+  WsArgs A{};
+  void *ctl = nullptr, *tasks = nullptr, *ready = nullptr;
+  A.units = units;
+  A.split = o->ws_split_threshold ? (uint64_t)(o->ws_split_threshold + 63) / 64 * 64 : kLftjSplit;
+  A.cap = kLftjTaskSlots;
+  AMJ_TRY(scratch(ctx, SLOT_DYN_CTL, sizeof(WsCtl), &ctl));
+  AMJ_TRY(scratch(ctx, SLOT_DYN_TASKS, A.cap * lftj_ws_task_bytes(), &tasks));
+  AMJ_TRY(scratch(ctx, SLOT_DYN_READY, A.cap * 4, &ready));
+  A.ctl = (WsCtl*)ctl;
+  A.tasks = (uint32_t*)tasks;
+  A.ready = (uint32_t*)ready;
+  int clock_khz = 0;
+  if (hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, ctx->device) !=
+          hipSuccess || clock_khz <= 0)
+    clock_khz = 100000;
+  A.stall_ticks = (o->ws_stall_ms ? (uint64_t)o->ws_stall_ms : kLftjStallMs) * (uint64_t)clock_khz;
+  const bool mat = o->materialize != 0;
+  const bool hash = o->checksum || !mat;
+  const int final_mode = mat ? (hash ? MODE_WRITE_HASH : MODE_WRITE) : MODE_HASH;
+  // persistent grid: the blocks the final pass's kernel fits on the device, at most one wave
+  // per unit
+  const uint64_t fit = (uint64_t)lftj_ws_blocks_per_cu(P, final_mode) *
+                       (uint64_t)ctx->props.multiProcessorCount;
+  const unsigned grid = (unsigned)std::max<uint64_t>(
+      1, std::min<uint64_t>((units + kWavesPerBlock - 1) / kWavesPerBlock, fit));
+  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[2], s));
+  const uint64_t* order = nullptr;
+  AMJ_TRY(unit_order(ctx, P, &order));
+  WsCtl hc{};
+  double kms = 0.0;
+  uint32_t passes = 0;
+  uint64_t pushed = 0;
+  if (!mat) {
+    AMJ_TRY(lftj_ws_pass(ctx, P, LOut{}, MODE_HASH, A, order, grid, &hc, &kms));
+    ++passes;
+    pushed = hc.pushed;
+    res->count = hc.count;
+    res->checksum = hc.csum;
+  } else {
+    uint64_t capacity = o->dro_capacity_rows;
+    if (capacity == 0 && o->dro) {
+      AMJ_TRY(sample_capacity(ctx, P, &capacity));
+    } else if (capacity == 0) {  // counting pass, then an exact allocation
+      AMJ_TRY(lftj_ws_pass(ctx, P, LOut{}, MODE_COUNT, A, order, grid, &hc, &kms));
+      ++passes;
+      pushed += hc.pushed;
+      capacity = hc.count;
+    }
+    uint64_t rows = 0;
+    for (int attempt = 0; attempt < 2; ++attempt) {
+      LOut out{};
+      if (capacity > 0) {
+        if (capacity * (uint64_t)P.n_attrs * 4 > budget_left(ctx, o)) {
+          set_error(ctx, AMJ_ERR_NOMEM, "result of " + std::to_string(capacity) +
+                                            " tuples does not fit the HBM budget (use "
+                                            "materialize=0)");
+          return AMJ_ERR_NOMEM;
+        }
+        AMJ_TRY(alloc_output(ctx, capacity, P.n_attrs, res));
+        for (int a = 0; a < P.n_attrs; ++a) out.c[a] = res->cols[a];
+      }
+      A.out_cap = capacity;
+      AMJ_TRY(lftj_ws_pass(ctx, P, out, final_mode, A, order, grid, &hc, &kms));
+      ++passes;
+      pushed += hc.pushed;
+      rows = hc.out_rows;
+      if (rows <= capacity) break;
+      amj_result_free(ctx, res);  // overran: the pass counted every result, run again exactly
+      capacity = rows;
+    }
+    if (rows > capacity) {
+      set_error(ctx, AMJ_ERR_STATE, "direct output overran an exact allocation");
+      return AMJ_ERR_STATE;
+    }
+    res->count = rows;
+    res->rows_written = rows;
+    res->checksum = hash ? hc.csum : 0;
+  }
+  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[5], s));
+  AMJ_HIP_TRY(ctx, hipEventSynchronize(ctx->ev[5]));
+  res->probe_ms = elapsed_ms(ctx->ev[2], ctx->ev[5]);
+  res->probe_kernel_ms = kms;
+  res->probe_kernel_launches = passes;
+  res->passes = passes;
+  res->detail = (uint32_t)std::min<uint64_t>(pushed, 0xffffffffull);
+  res->path = AMJ_PATH_LFTJ_DFS;
+  return AMJ_OK;
+}
+
 static int run_lftj_dfs(amj_ctx_t* ctx, const LPlan& P, const amj_opts_t* o, amj_result_t* res) {
+  if (o->ws) return run_lftj_ws(ctx, P, o, res);
   if (o->dro && o->materialize) return run_lftj_direct(ctx, P, o, res);
   if (!o->materialize) return run_lftj_count(ctx, P, res);
   hipStream_t s = ctx->stream;
@@ -238,8 +371,6 @@ static int run_lftj_dfs(amj_ctx_t* ctx, const LPlan& P, const amj_opts_t* o, amj
   void *counts = nullptr, *tmp = nullptr, *bsum = nullptr;
   AMJ_TRY(scratch(ctx, SLOT_UNIT_COUNTS, (units + 1) * 8, &counts));
   AMJ_TRY(scratch(ctx, SLOT_SCAN_TEMP, scan_temp_elems(units + 1) * 8, &tmp));
-  // opts.ws runs this same engine: every intersection is already spread over the 64 lanes of
-  // a wave (what the reference's ws buys by splitting long intersections)
   AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[2], s));
   const uint64_t* order = nullptr;
   AMJ_TRY(unit_order(ctx, P, &order));

@@ -82,15 +82,22 @@ __device__ __forceinline__ void make_child(const Plan& P, const Lv& par,
   finish_level(ch);
 }
 
+// Called once per 64 results by the emitters below (the persistent engine counts its steps).
+struct NoStep {
+  __device__ __forceinline__ void operator()() const {}
+};
+
 // Emit the results [0, T) of the last level at absolute rows out_row0 + e.
-template <bool MATERIALIZE, bool HASH>
+template <bool MATERIALIZE, bool HASH, typename Step = NoStep>
 __device__ __forceinline__ void emit(const Plan& P, int t, const Lv& l,
-                                     uint64_t out_row0, const ProbeOut& out, Acc& acc) {
+                                     uint64_t out_row0, const ProbeOut& out, Acc& acc,
+                                     const Step& step = Step{}) {
   const bool bind = P.val_mode[t] == VAL_BIND;
   const int vattr = P.val_attr[t];
   const int32_t* vals = P.tab[t].vals;
   const uint64_t e1 = l.T;
   for (uint64_t c = 0; c < e1; c += kWave) {
+    step();
     const uint64_t g = c + (uint64_t)lane_id();
     const bool valid = g < e1;
     const uint64_t gg = valid ? g : e1 - 1;
@@ -150,15 +157,17 @@ __device__ __forceinline__ void direct_flush(const ProbeOut& out, int na, Direct
 }
 
 // Results [0, T) of a last level into the wave's buffer (MAT) and the checksum (HASH).
-template <bool MAT, bool HASH>
+template <bool MAT, bool HASH, typename Step = NoStep>
 __device__ __forceinline__ void direct_emit(const Plan& P, int t, const Lv& l,
                                             const ProbeOut& out, Acc& acc, DirectStage& st,
-                                            unsigned long long* rows) {
+                                            unsigned long long* rows,
+                                            const Step& step = Step{}) {
   const bool bind = P.val_mode[t] == VAL_BIND;
   const int vattr = P.val_attr[t];
   const int32_t* vals = P.tab[t].vals;
   const uint64_t e1 = l.T;
   for (uint64_t c = 0; c < e1; c += kWave) {
+    step();
     const uint64_t g = c + (uint64_t)lane_id();
     const bool valid = g < e1;
     const uint64_t gg = valid ? g : e1 - 1;

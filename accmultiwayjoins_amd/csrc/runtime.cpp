@@ -49,13 +49,14 @@ void dfree(amj_ctx_t* ctx, void* p, size_t bytes) {
 int scratch(amj_ctx_t* ctx, int slot, size_t bytes, void** out) {
   auto& b = ctx->scratch[slot];
   if (b.bytes < bytes) {
+    const size_t old = b.bytes;  // grow by at least half, so growing requests rarely reallocate
     if (b.p) {
       AMJ_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
       dfree(ctx, b.p, b.bytes);
       b.p = nullptr;
       b.bytes = 0;
     }
-    size_t want = std::max(bytes, b.bytes + b.bytes / 2);
+    size_t want = std::max(bytes, old + old / 2);
     int rc = dmalloc(ctx, &b.p, want);
     if (rc != AMJ_OK) {
       rc = dmalloc(ctx, &b.p, bytes);

@@ -145,7 +145,7 @@ int finish_checksum(amj_ctx_t* ctx, const uint64_t* block_sums, uint64_t nb, uin
 uint64_t budget_left(amj_ctx_t* ctx, const amj_opts_t* o);
 // Frequent-item path: suffix tables + output-window writer. AMJ_ERR_STATE asks the caller to
 // use the count/write path instead (a frontier level would exceed 2^32 children).
-int run_fib(amj_ctx_t* ctx, Plan& P, const amj_opts_t* o, amj_result_t* res);
+int run_fib(amj_ctx_t* ctx, const Plan& plan, const amj_opts_t* o, amj_result_t* res);
 // Plan of exec-AMHJ TRI / FOUR over one edge relation and its table (built with sort_vals=1).
 int make_pattern_plan(amj_ctx_t* ctx, const amj_relation* edges, const amj_hash_table* ht,
                       int query, Plan* plan, std::vector<uint32_t>* ids);

@@ -72,7 +72,10 @@ typedef struct {
 /* Query options: the exec-AMHJ switches (exec_AMHJ.cu:216-245) plus this library's knobs. */
 typedef struct {
   int ooc;                      /* stream the probe relation in slices (ooc.cuh:166-306) */
-  int ws;                       /* work sharing between waves */
+  int ws;                       /* work sharing between waves: the persistent AMHJ engine
+                                   (long runs of a frontier level shared as tasks), and ALFTJ
+                                   DFS (long candidate runs shared as tasks, ALFTJ.cuh:464-499);
+                                   BFS and out-of-core ALFTJ do not use it */
   int dro;                      /* direct result output (single pass, no count pass): AMHJ,
                                    ALFTJ DFS and batches; row order then depends on timing */
   int fib;                      /* frequent-item buffer (subtree counts of join values) */
@@ -91,7 +94,12 @@ typedef struct {
   uint32_t engine;              /* AMJ_ENGINE_WAVE (0): the wave engines above;
                                    AMJ_ENGINE_TPT: thread-per-tuple baseline (MHJ_count /
                                    MHJ_write, MHJ-GPU/Joins/MHJ.cuh:17-321), for comparison */
-  uint32_t reserved[4];
+  uint32_t ws_stall_ms;         /* ws (AMHJ and ALFTJ persistent engines): a launch in which no
+                                   wave reports progress for this long stops with
+                                   AMJ_ERR_STATE instead of hanging (0: 20,000 ms); working
+                                   waves report progress as they go, so a slow pass is not
+                                   mistaken for a stuck one */
+  uint32_t reserved[3];
 } amj_opts_t;
 #define AMJ_ENGINE_WAVE 0
 #define AMJ_ENGINE_TPT 1

@@ -234,6 +234,28 @@ def test_dynamic_paths_on_skew_and_patterns(ctx, opts):
     assert (r4.count, r4.checksum) == (n4, c4)
 
 
+def test_ws_slow_pass_with_progress_is_not_a_stall(ctx):
+    # One probe row opens 4,096 rows of R2 that each lead to 4,096 rows of R3: one wave walks
+    # 16.7 M last-level results alone (the last level is never split) while every other wave
+    # is idle. With a 50 ms stall limit the idle waves would give up if only completed units and
+    # tasks counted as progress; the working wave's heartbeat keeps the pass alive, and its
+    # count and checksum equal the oracle's.
+    rng = np.random.default_rng(11)
+    n = 20000
+    b = rng.integers(100, 5000, n).astype(np.int32)
+    b[1234] = 7
+    r2b = np.concatenate([np.full(4096, 7, np.int32), rng.integers(100, 5000, 3000).astype(np.int32)])
+    r2c = np.concatenate([np.full(4096, 9, np.int32), rng.integers(100, 5000, 3000).astype(np.int32)])
+    r3c = np.concatenate([np.full(4096, 9, np.int32), rng.integers(100, 5000, 3000).astype(np.int32)])
+    r3d = rng.integers(0, 1 << 30, len(r3c)).astype(np.int32)
+    rels = [([rng.integers(0, 1 << 30, n).astype(np.int32), b], [0, 1]),
+            ([r2b, r2c], [1, 2]), ([r3c, r3d], [2, 3])]
+    want_cnt, want_cs, _, _, _ = O.mhj(rels)
+    assert want_cnt > 4096 * 4096
+    r = ctx.chain_join(rels, materialize=False, ws=1, ws_stall_ms=50)
+    assert (r.count, r.checksum) == (want_cnt, want_cs) and r.path == "ws"
+
+
 def test_autotune_choice_gives_the_same_result(ctx):
     rels = G.chain_relations([1 << 20, 1 << 18, 1 << 18, 1 << 18], 1 << 16, seed=42, zipf_s=1.0,
                              payload_range=1 << 31)

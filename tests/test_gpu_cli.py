@@ -70,9 +70,7 @@ def test_exec_amhj_tri_four(tmp_path):
     assert int(field(out, r"checksum=0x([0-9a-f]+)"), 16) == c4
 
 
-@pytest.mark.parametrize("algo", ["0", pytest.param("1", marks=pytest.mark.skip(
-    reason="faulted with ws=1 in the taken-out ALFTJ ws kernels; DFS from a .tr file is "
-           "covered with ws=0 below"))])
+@pytest.mark.parametrize("algo", ["0", "1"])
 def test_exec_lftj_self_db_and_tr(ctx, tmp_path, algo):
     s, d = G.rmat_oriented(11, 16, seed=9)
     n3, c3 = O.triangles(s, d)
@@ -89,6 +87,7 @@ def test_exec_lftj_self_db_and_tr(ctx, tmp_path, algo):
     out = run(["exec-LFTJ", str(b), algo, "0", "1"])
     assert "Read from Trie file" in out
     assert int(field(out, r"Output count: (\d+)")) == n3
+    assert int(field(out, r"checksum=0x([0-9a-f]+)"), 16) == c3
 
 
 def test_exec_lftj_dfs_from_tr_file(ctx, tmp_path):

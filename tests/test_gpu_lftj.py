@@ -160,18 +160,51 @@ def test_attr_order_errors(ctx):
     assert ctx.lftj([t01, t01], [0, 1]).count == 2
 
 
-@pytest.mark.skip(reason="the persistent unit-claiming ALFTJ ws kernels faulted on their first "
-                         "GPU run (exec-LFTJ DFS ws=1, R-MAT scale 11); cause not found, the "
-                         "kernels were taken out and ws=1 runs the unit engine")
 @pytest.mark.parametrize("query", ["TRI", "FOUR"])
-def test_work_sharing_dfs_same_arrays(ctx, query):
+@pytest.mark.parametrize("split", [0, 64], ids=["split1024", "split64"])
+@pytest.mark.parametrize("mode", ["count", "write", "dro", "dro-overrun"])
+def test_work_sharing_dfs_matches_oracle(ctx, query, split, mode):
+    # ws=1: persistent waves, candidate runs longer than the split shared as tasks
+    # (ALFTJ.cuh:464-499). Count + checksum equal the oracle's; written results are the same
+    # multiset of tuples as the unit engine's (direct output: the row order is not fixed).
     s, d = G.rmat_oriented(11, 16, seed=3)
-    a = ctx.lftj_pattern(s, d, query, "DFS", fetch=True)
-    b = ctx.lftj_pattern(s, d, query, "DFS", fetch=True, ws=1)
-    assert (a.count, a.checksum) == (b.count, b.checksum)
-    # units keep their scanned output rows whichever wave takes them
-    for x, y in zip(a.cols, b.cols):
-        assert np.array_equal(x, y)
+    want = O.triangles(s, d) if query == "TRI" else O.four_cliques(s, d)
+    kw = dict(ws=1, ws_split_threshold=split)
+    if mode == "count":
+        r = ctx.lftj_pattern(s, d, query, "DFS", materialize=False, **kw)
+        assert (r.count, r.checksum) == want and r.rows_written == 0
+    else:
+        if mode != "write":
+            kw.update(dro=1, dro_capacity_rows=16 if mode == "dro-overrun" else 0)
+        r = ctx.lftj_pattern(s, d, query, "DFS", fetch=True, **kw)
+        assert (r.count, r.checksum) == want and r.rows_written == r.count
+        base = ctx.lftj_pattern(s, d, query, "DFS", fetch=True)
+        assert _tuples(r) == _tuples(base)
+        if mode == "dro-overrun":
+            assert r.passes == 2  # the first pass overran its 16 rows and counted everything
+    assert r.path == "lftj-dfs"
+    if split == 64:
+        assert r.detail > 0  # tasks were shared between waves
+
+
+def test_work_sharing_dfs_rmat18_matches_oracle(ctx):
+    # the skewed R-MAT 18 graph (hub vertices with thousands of oriented edges): default split,
+    # heaviest-first unit order, triangles counted and written, 4-cliques counted
+    s, d = G.rmat_oriented(18, 16, seed=42)
+    n3, c3 = O.triangles(s, d)
+    r = ctx.lftj_pattern(s, d, "TRI", "DFS", materialize=False, ws=1)
+    assert (r.count, r.checksum) == (n3, c3)
+    assert r.detail > 0
+    w = ctx.lftj_pattern(s, d, "TRI", "DFS", materialize=True, ws=1)
+    assert (w.count, w.checksum, w.rows_written) == (n3, c3, n3)
+    md5 = ctx.md5(w)
+    ctx.free(w)
+    u = ctx.lftj_pattern(s, d, "TRI", "DFS", materialize=True)
+    assert ctx.md5(u) == md5  # the same multiset of tuples as the unit engine
+    ctx.free(u)
+    h4 = ctx.pattern_join(s, d, "FOUR", materialize=False)
+    l4 = ctx.lftj_pattern(s, d, "FOUR", "DFS", materialize=False, ws=1)
+    assert (l4.count, l4.checksum) == (h4.count, h4.checksum)
 
 
 @pytest.mark.parametrize("query", ["TRI", "FOUR"])
