@@ -101,6 +101,16 @@ SIGNATURES = [
     ("amj_execute", C.c_int,
      [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(AmjOpts),
       C.POINTER(AmjResult)]),
+    ("amj_device_alloc", C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_void_p)]),
+    ("amj_device_free", None, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    ("amj_memcpy", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
+    ("amj_gen_device_uniform", C.c_int,
+     [C.c_void_p, C.c_void_p, C.c_uint64, C.c_int64, C.c_int64, C.c_uint64, C.c_uint64]),
+    ("amj_gen_device_zipf", C.c_int,
+     [C.c_void_p, C.c_void_p, C.c_uint64, C.c_uint64, C.c_double, C.c_uint64, C.c_uint64]),
+    ("amj_gen_device_rmat", C.c_int,
+     [C.c_void_p, C.c_int, C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_uint64,
+      C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64)]),
     ("amj_chain_join", C.c_int,
      [C.c_void_p, C.POINTER(AmjRel), C.c_uint32, C.POINTER(AmjOpts), C.POINTER(AmjResult)]),
     ("amj_pattern_join", C.c_int,

@@ -152,6 +152,31 @@ int amj_device_info(amj_ctx_t* ctx, char* name, size_t name_len, uint64_t* hbm_b
 void amj_default_opts(amj_opts_t* opts);
 const char* amj_version(void);
 
+/* Device memory for caller-owned columns (relations created with on_device = 1 from them):
+ * counted in amj_memory_info; free with the same size. */
+int amj_device_alloc(amj_ctx_t* ctx, size_t bytes, void** out);
+void amj_device_free(amj_ctx_t* ctx, void* p, size_t bytes);
+/* Copy bytes between any host and device memory, on the context's stream (synchronous). */
+int amj_memcpy(amj_ctx_t* ctx, void* dst, const void* src, size_t bytes);
+/* Synthetic int32 columns written on the device (no counterpart in the reference, whose
+ * generators write files and are not seeded reproducibly). Value i is a pure function of
+ * (seed, stream, i). uniform: over [lo, hi), the same values as the host amj_gen_uniform.
+ * zipf: Zipf(s) over k keys (0 <= s <= 16, k <= 2^31) by rejection-inversion sampling, rank r
+ * mapped to key ((r - 1) * a + b) mod k by a seeded bijection. Synchronous on the context's
+ * stream. */
+int amj_gen_device_uniform(amj_ctx_t* ctx, int32_t* out, uint64_t n, int64_t lo, int64_t hi,
+                           uint64_t seed, uint64_t stream);
+int amj_gen_device_zipf(amj_ctx_t* ctx, int32_t* out, uint64_t n, uint64_t k, double s,
+                        uint64_t seed, uint64_t stream);
+/* R-MAT graph on the device: edge_factor << scale edges with quadrant probabilities (a, b, c),
+ * vertices relabelled by a seeded permutation, then symmetrised, self-loops and duplicates
+ * dropped, oriented u -> v iff (deg u, u) < (deg v, v) and sorted by (src, dst): the same edges
+ * as the host generator (amj_gen_rmat + amj_graph_orient in amj_gen.h). out_src and out_dst
+ * are device memory of edge_factor << scale int32 each; *edges gets the edges kept. */
+int amj_gen_device_rmat(amj_ctx_t* ctx, int scale, uint32_t edge_factor, double a, double b,
+                        double c, uint64_t seed, int32_t* out_src, int32_t* out_dst,
+                        uint64_t* edges);
+
 /* ---- relations and indexes ---------------------------------------------------------- */
 int amj_relation_create(amj_ctx_t* ctx, const amj_rel_t* rel, amj_relation_t** out);
 void amj_relation_destroy(amj_ctx_t* ctx, amj_relation_t* rel);
