@@ -111,6 +111,9 @@ SIGNATURES = [
     ("amj_gen_device_rmat", C.c_int,
      [C.c_void_p, C.c_int, C.c_uint32, C.c_double, C.c_double, C.c_double, C.c_uint64,
       C.c_void_p, C.c_void_p, C.POINTER(C.c_uint64)]),
+    ("amj_execute_noalloc", C.c_int,
+     [C.c_void_p, C.c_void_p, C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(AmjOpts),
+      C.POINTER(AmjResult)]),
     ("amj_chain_join", C.c_int,
      [C.c_void_p, C.POINTER(AmjRel), C.c_uint32, C.POINTER(AmjOpts), C.POINTER(AmjResult)]),
     ("amj_pattern_join", C.c_int,

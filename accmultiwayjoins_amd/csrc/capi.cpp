@@ -42,6 +42,34 @@ int make_pattern_plan(amj_ctx_t* ctx, const amj_relation* R, const amj_hash_tabl
 
 }  // namespace amj
 
+static TableUse use_of(const amj_hash_table* ht) {
+  TableUse u{};
+  u.ht = ht;
+  u.key_attr_id = ht->rel->attr_ids[ht->key_col];
+  u.val_attr_id = ht->rel->num_cols > 1 ? (int64_t)ht->rel->attr_ids[1 - ht->key_col] : -1;
+  return u;
+}
+
+namespace amj {
+
+int make_execute_plan(amj_ctx_t* ctx, const amj_relation* probe, amj_hash_table* const* hts,
+                      uint32_t n_hts, Plan* plan, std::vector<uint32_t>* ids) {
+  ProbeUse pu{};
+  pu.rel = probe;
+  for (uint32_t c = 0; c < probe->num_cols; ++c) pu.attr_ids[c] = probe->attr_ids[c];
+  std::vector<TableUse> uses;
+  for (uint32_t i = 0; i < n_hts; ++i) {
+    if (!hts[i]) {
+      set_error(ctx, AMJ_ERR_ARG, "a hash table pointer is NULL");
+      return AMJ_ERR_ARG;
+    }
+    uses.push_back(use_of(hts[i]));
+  }
+  return make_plan(ctx, pu, uses, plan, ids, nullptr);
+}
+
+}  // namespace amj
+
 extern "C" {
 
 const char* amj_version(void) { return "amj 0.1 (gfx950)"; }
@@ -204,28 +232,14 @@ int amj_hash_table_info(amj_ctx_t* ctx, const amj_hash_table_t* ht, uint64_t* ro
   return AMJ_OK;
 }
 
-static TableUse use_of(const amj_hash_table* ht) {
-  TableUse u{};
-  u.ht = ht;
-  u.key_attr_id = ht->rel->attr_ids[ht->key_col];
-  u.val_attr_id = ht->rel->num_cols > 1 ? (int64_t)ht->rel->attr_ids[1 - ht->key_col] : -1;
-  return u;
-}
+
 
 int amj_execute(amj_ctx_t* ctx, const amj_relation_t* probe, amj_hash_table_t* const* hts,
                 uint32_t n_hts, const amj_opts_t* opts, amj_result_t* res) {
   if (!ctx || !probe || !hts || !res) return AMJ_ERR_ARG;
-  ProbeUse pu{};
-  pu.rel = probe;
-  for (uint32_t c = 0; c < probe->num_cols; ++c) pu.attr_ids[c] = probe->attr_ids[c];
-  std::vector<TableUse> uses;
-  for (uint32_t i = 0; i < n_hts; ++i) {
-    if (!hts[i]) return AMJ_ERR_ARG;
-    uses.push_back(use_of(hts[i]));
-  }
   Plan plan;
   std::vector<uint32_t> ids;
-  AMJ_TRY(make_plan(ctx, pu, uses, &plan, &ids, nullptr));
+  AMJ_TRY(make_execute_plan(ctx, probe, hts, n_hts, &plan, &ids));
   return execute_plan(ctx, plan, ids, opts, res);
 }
 

@@ -134,6 +134,9 @@ int build_hash_table(amj_ctx_t* ctx, const amj_relation* rel, uint32_t key_col, 
 
 int execute_plan(amj_ctx_t* ctx, Plan& plan, const std::vector<uint32_t>& attr_ids,
                  const amj_opts_t* opts, amj_result_t* res);
+// Plan of amj_execute: probe relation and hash tables as given.
+int make_execute_plan(amj_ctx_t* ctx, const amj_relation* probe, amj_hash_table* const* hts,
+                      uint32_t n_hts, Plan* plan, std::vector<uint32_t>* ids);
 
 uint32_t floor_log2_u64(uint64_t v);
 double elapsed_ms(hipEvent_t a, hipEvent_t b);

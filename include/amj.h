@@ -206,6 +206,17 @@ int amj_pattern_join(amj_ctx_t* ctx, const amj_rel_t* edges, int query, const am
  * resident counterpart of amj_pattern_join (build_hash_single, QueryProcessor.cuh:112-169). */
 int amj_execute_pattern(amj_ctx_t* ctx, const amj_relation_t* edges, const amj_hash_table_t* ht,
                         int query, const amj_opts_t* opts, amj_result_t* res);
+/* amj_execute's count + checksum (materialize = 0) with no device allocation. No counterpart
+ * in the reference: for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Runs the plain engine's
+ * one-pass count-and-hash launch with its temporaries in a fixed 8.4 MB device array (held on
+ * every GPU the library runs on until the process ends); the probe relation must be resident
+ * and hold at most 2^27 rows (AMJ_ERR_ARG beyond). opts may be NULL; otherwise materialize,
+ * ooc, ws, dro and fib must be 0 and the engine AMJ_ENGINE_WAVE. Calls that overlap on one GPU
+ * share that array and race. res gets count and checksum (no columns). */
+int amj_execute_noalloc(amj_ctx_t* ctx, const amj_relation_t* probe,
+                        amj_hash_table_t* const* hts, uint32_t n_hts, const amj_opts_t* opts,
+                        amj_result_t* res);
 int amj_result_free(amj_ctx_t* ctx, amj_result_t* res);
 /* Copy rows [row0, row0+n) of result column `col` to host memory. */
 int amj_result_copy_col(amj_ctx_t* ctx, const amj_result_t* res, uint32_t col, uint64_t row0,
