@@ -20,7 +20,7 @@ STATUS = {
     -4: "AMJ_ERR_NODEV", -5: "AMJ_ERR_STATE",
 }
 PATHS = {1: "plain", 2: "fib", 3: "ws", 4: "dro", 5: "lftj-dfs", 6: "lftj-bfs", 7: "ws-dro", 8: "batch",
-         9: "ooc", 10: "lftj-ooc", 11: "tpt"}
+         9: "ooc", 10: "lftj-ooc", 11: "tpt", 12: "open-addr", 13: "pw"}
 
 QUERY = {"NORMAL": 0, "TRI": 1, "FOUR": 2}
 AMJ_REL_HOST = 2
@@ -54,7 +54,8 @@ class AmjOpts(C.Structure):
         ("probe_order", C.c_int),
         ("engine", C.c_uint32),
         ("ws_stall_ms", C.c_uint32),
-        ("reserved", C.c_uint32 * 3),
+        ("ws_claim_units", C.c_uint32),
+        ("reserved", C.c_uint32 * 2),
     ]
 
 
@@ -136,6 +137,8 @@ SIGNATURES = [
      [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.POINTER(AmjOpts), C.POINTER(C.c_uint32)]),
     ("amj_batch_set_budget", C.c_int, [C.c_void_p, C.c_uint64]),
     ("amj_batch_run", C.c_int, [C.c_void_p]),
+    ("amj_batch_capture", C.c_int, [C.c_void_p, C.c_uint64]),
+    ("amj_batch_replay", C.c_int, [C.c_void_p]),
     ("amj_batch_result", C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(AmjResult)]),
     ("amj_batch_info", C.c_int,
      [C.c_void_p, C.POINTER(C.c_uint32), C.POINTER(C.c_double), C.POINTER(C.c_uint64)]),
@@ -150,6 +153,15 @@ SIGNATURES = [
      [C.c_void_p, C.c_uint32, C.c_int, C.c_int, C.POINTER(C.c_uint64)]),
     ("amj_server_wait", C.c_int,
      [C.c_void_p, C.c_uint64, C.POINTER(AmjResult), C.POINTER(C.c_double)]),
+    ("amj_server_wait_ex", C.c_int,
+     [C.c_void_p, C.c_uint64, C.POINTER(AmjResult), C.POINTER(C.c_double),
+      C.POINTER(C.c_double)]),
+    ("amj_server_set_budget", C.c_int, [C.c_void_p, C.c_uint64]),
+    ("amj_server_set_autotune", C.c_int, [C.c_void_p, C.c_uint64]),
+    ("amj_server_tuning", C.c_int,
+     [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    ("amj_server_memory", C.c_int,
+     [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     ("amj_server_release", C.c_int, [C.c_void_p, C.POINTER(AmjResult)]),
     ("amj_server_copy_col", C.c_int,
      [C.c_void_p, C.POINTER(AmjResult), C.c_uint32, C.c_uint64, C.c_uint64, C.c_void_p]),

@@ -54,7 +54,8 @@ class JoinResult:
 
 def _opts(ooc=0, ws=0, dro=0, fib=0, materialize=True, checksum=True, hbm_budget_bytes=0,
           ooc_slice_rows=0, out_chunk_rows=0, dro_capacity_rows=0, probe_order=0,
-          ws_split_threshold=0, fib_suffix=0, engine="wave", ws_stall_ms=0) -> N.AmjOpts:
+          ws_split_threshold=0, fib_suffix=0, engine="wave", ws_stall_ms=0,
+          ws_claim_units=0) -> N.AmjOpts:
     o = N.AmjOpts()
     N.lib().amj_default_opts(C.byref(o))
     o.ooc, o.ws, o.dro, o.fib = int(ooc), int(ws), int(dro), int(fib)
@@ -68,7 +69,8 @@ def _opts(ooc=0, ws=0, dro=0, fib=0, materialize=True, checksum=True, hbm_budget
     o.fib_suffix = int(fib_suffix)
     o.probe_order = int(probe_order)
     o.ws_stall_ms = int(ws_stall_ms)
-    engines = {"wave": 0, "tpt": 1}
+    o.ws_claim_units = int(ws_claim_units)
+    engines = {"wave": 0, "tpt": 1, "open-addr": 2, "pw": 3}
     if engine not in engines:
         raise ValueError(f"engine must be one of {sorted(engines)}")
     o.engine = engines[engine]

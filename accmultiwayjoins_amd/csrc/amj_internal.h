@@ -255,9 +255,10 @@ uint64_t probe_write_blocks(const Plan& plan);
 // counts[0 .. tpt_blocks * kBlock); the write pass writes lane g's results from base[g] on.
 // With hash, per-block checksum partials go to block_sums[0 .. tpt_blocks).
 uint64_t tpt_blocks(const Plan& plan);
+struct OaTables;
 hipError_t launch_tpt(const Plan& plan, bool write, bool hash, uint64_t* counts,
                       const uint64_t* base, const ProbeOut& out, uint64_t* block_sums,
-                      hipStream_t s);
+                      hipStream_t s, const OaTables* oa = nullptr);
 
 // Frequent-item (subtree-count) buffer for linear chains.
 hipError_t launch_fib_weights(const Plan& plan, int level, uint64_t* w, RunRef* next,

@@ -2,10 +2,13 @@
 // relation, and remembers the choice per query shape in the context.
 //
 // Knobs (the reference fixes all of them at compile time, MHJ-GPU/conf.h): the engine (count /
-// write unit engine, persistent ws engine, dro), the ws task size, and for frequent-item chains
-// the output-window size and the suffix-table mode. Candidates run on an evenly strided sample
-// of at most 4 M probe rows with the caller's materialise setting; the fastest (device time,
-// best of two runs) wins.
+// write unit engine, persistent ws engine, dro), the ws task size (how many waves share a heavy
+// key's run) and claim size (units a wave takes at once), and for frequent-item chains the
+// output-window size (rows per wave step) and the suffix-table mode. Candidates run on an evenly
+// strided sample of at most 4 M probe rows with the caller's materialise setting; the fastest
+// (device time, best of two runs) wins and is remembered per query shape. The hash tables'
+// bucket count stays the reference's rule (B = floorPow2(rows), QueryProcessor.cuh:86), which
+// the tables were built with before the query.
 #include <cstring>
 
 #include "runtime.h"
@@ -33,9 +36,26 @@ int amj_autotune(amj_ctx_t* ctx, const amj_relation_t* probe, amj_hash_table_t* 
     set_error(ctx, AMJ_ERR_ARG, "autotune: the probe relation must be on the device");
     return AMJ_ERR_ARG;
   }
-  std::vector<uint64_t> key{(uint64_t)(uintptr_t)probe, probe->rows,
-                            (uint64_t)(opts->materialize != 0)};
-  for (uint32_t i = 0; i < n_hts; ++i) key.push_back((uint64_t)(uintptr_t)hts[i]);
+  Plan full;
+  std::vector<uint32_t> full_ids;
+  AMJ_TRY(make_execute_plan(ctx, probe, hts, n_hts, &full, &full_ids));
+  // the query shape: depth, attributes, and per level the size class of the table, its bucket
+  // count, whether it is an identity table and what the level does with the second column,
+  // with the probe's size class; queries of one shape share a choice (relations freed and
+  // reallocated at the same address are not mistaken for the ones tuned before)
+  auto log2c = [](uint64_t x) {
+    uint64_t b = 0;
+    while ((1ull << b) < x && b < 63) ++b;
+    return b;
+  };
+  std::vector<uint64_t> key{(uint64_t)(opts->materialize != 0), (uint64_t)full.n_levels,
+                            (uint64_t)full.n_attrs, log2c(full.probe_n)};
+  for (int t = 0; t < full.n_levels; ++t) {
+    key.push_back(log2c(full.tab[t].n));
+    key.push_back(full.tab[t].log_b);
+    key.push_back(full.tab[t].identity ? 1 : 0);
+    key.push_back((uint64_t)full.val_mode[t]);
+  }
   auto& cache = ctx->tuned;
   auto hit = cache.find(key);
   if (hit != cache.end()) {
@@ -64,11 +84,14 @@ int amj_autotune(amj_ctx_t* ctx, const amj_relation_t* probe, amj_hash_table_t* 
     sample.rows = ns;
     AMJ_HIP_TRY(ctx, hipGetLastError());
   }
-  // candidates
+  // candidates: the engine, the persistent engine's task size (waves per heavy key) and claim
+  // size (units per claim), direct output, and for frequent-item chains the output window
+  // (rows per wave step) and the suffix tables
   std::vector<amj_opts_t> cands;
   amj_opts_t base = *opts;
   base.ooc = 0;
-  auto add = [&](int ws, int dro, int fib, uint64_t window, int suffix, uint32_t split) {
+  auto add = [&](int ws, int dro, int fib, uint64_t window, int suffix, uint32_t split,
+                 uint32_t claim) {
     amj_opts_t c = base;
     c.ws = ws;
     c.dro = dro;
@@ -76,32 +99,16 @@ int amj_autotune(amj_ctx_t* ctx, const amj_relation_t* probe, amj_hash_table_t* 
     c.out_chunk_rows = window;
     c.fib_suffix = suffix;
     c.ws_split_threshold = split;
+    c.ws_claim_units = claim;
     cands.push_back(c);
   };
-  add(0, 0, 0, 0, 0, 0);                  // unit engine
-  add(1, 0, 0, 0, 0, 1024);               // persistent engine, small tasks
-  add(1, 0, 0, 0, 0, 8192);               // persistent engine, large tasks
-  if (base.materialize) add(0, 1, 0, 0, 0, 0);  // direct output
-  {
-    ProbeUse pu{};
-    pu.rel = &sample;
-    for (uint32_t c = 0; c < sample.num_cols; ++c) pu.attr_ids[c] = sample.attr_ids[c];
-    std::vector<TableUse> uses;
-    for (uint32_t i = 0; i < n_hts; ++i) {
-      TableUse u{};
-      u.ht = hts[i];
-      u.key_attr_id = hts[i]->rel->attr_ids[hts[i]->key_col];
-      u.val_attr_id =
-          hts[i]->rel->num_cols > 1 ? (int64_t)hts[i]->rel->attr_ids[1 - hts[i]->key_col] : -1;
-      uses.push_back(u);
-    }
-    Plan P;
-    std::vector<uint32_t> ids;
-    AMJ_TRY(make_plan(ctx, pu, uses, &P, &ids, nullptr));
-    if (fib_eligible(P) && window_plan_supported(P))
-      for (uint64_t w : {1024ull, 4096ull, 16384ull})
-        for (int sfx : {0, 1}) add(1, 1, 1, w, sfx, 0);
-  }
+  add(0, 0, 0, 0, 0, 0, 0);                    // unit engine
+  for (uint32_t split : {1024u, 8192u})
+    for (uint32_t claim : {4u, 64u}) add(1, 0, 0, 0, 0, split, claim);  // persistent engine
+  if (base.materialize) add(0, 1, 0, 0, 0, 0, 0);  // direct output
+  if (fib_eligible(full) && window_plan_supported(full))
+    for (uint64_t w : {1024ull, 4096ull, 16384ull})
+      for (int sfx : {0, 1}) add(1, 1, 1, w, sfx, 0, 0);
   double best = 1e300;
   amj_opts_t pick = base;
   for (const auto& c : cands) {

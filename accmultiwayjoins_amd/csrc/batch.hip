@@ -231,6 +231,88 @@ hipError_t launch_batch_totals(const void* qs, uint32_t nq, const uint64_t* base
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------------------
+// Captured batches: output sizes and places decided on the device, so the whole chain (size
+// sample -> sizing and placement -> direct output) runs without a host round trip and can be
+// recorded into a HIP graph.
+// ---------------------------------------------------------------------------------------
+struct PrepInfo {
+  uint64_t units;     // units of the query
+  uint64_t sunits;    // units sampled (0: no sample: count-only, or a size given)
+  uint64_t user_cap;  // output rows given by the caller (0: from the sample)
+  uint32_t n_attrs;
+  uint32_t mat;       // materialised
+};
+
+// Per query k (in launch order): its output size (exact when every unit was sampled, else the
+// sample scaled up by 1.25 plus 64 Ki rows), its place in the slab (column-major, 256-byte
+// aligned, in launch order), and whether it fits; the direct-output descriptor gets its
+// columns and capacity (none when it does not fit: it is counted, not written). One thread:
+// a batch holds at most a few thousand queries.
+__global__ void batch_place_kernel(const PrepInfo* __restrict__ info, uint32_t nq,
+                                   const uint64_t* __restrict__ sampled, BatchQ* __restrict__ dq,
+                                   int32_t* __restrict__ slab, uint64_t slab_words,
+                                   uint64_t* __restrict__ cap_out, uint64_t* __restrict__ off_out,
+                                   uint32_t* __restrict__ admitted) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  uint64_t off = 0;
+  for (uint32_t k = 0; k < nq; ++k) {
+    const PrepInfo& I = info[k];
+    uint64_t cap = 0;
+    if (I.mat) {
+      if (I.user_cap) cap = I.user_cap;
+      else if (I.sunits == I.units) cap = sampled[k];
+      else cap = (uint64_t)((double)sampled[k] * (double)I.units / (double)I.sunits * 1.25) + 65536;
+    }
+    const uint64_t words = (cap * I.n_attrs + 63) / 64 * 64;
+    const bool fits = !I.mat || off + words <= slab_words;
+    for (int a = 0; a < kMaxAttrs; ++a)
+      dq[k].out.cols[a] = (fits && cap && a < (int)I.n_attrs) ? slab + off + (uint64_t)a * cap
+                                                              : nullptr;
+    dq[k].out.capacity = fits ? cap : 0;
+    cap_out[k] = cap;
+    off_out[k] = fits ? off : 0;
+    admitted[k] = fits ? 1u : 0u;
+    if (fits) off += words;
+  }
+}
+
+// Clears counters inside a captured chain (a kernel, not a memset node).
+__global__ void fill_u64_kernel(uint64_t* __restrict__ p, uint64_t n, uint64_t v) {
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (uint64_t)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+hipError_t launch_batch_place(const void* info, uint32_t nq, const uint64_t* sampled, void* dq,
+                              int32_t* slab, uint64_t slab_words, uint64_t* cap_out,
+                              uint64_t* off_out, uint32_t* admitted, hipStream_t s) {
+  if (nq == 0) return hipSuccess;
+  hipLaunchKernelGGL(batch_place_kernel, dim3(1), dim3(64), 0, s, (const PrepInfo*)info, nq,
+                     sampled, (BatchQ*)dq, slab, slab_words, cap_out, off_out, admitted);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  uint64_t g = (n + 255) / 256;
+  if (g > 4096) g = 4096;
+  hipLaunchKernelGGL(fill_u64_kernel, dim3((unsigned)g), dim3(256), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+size_t batch_prep_info_bytes() { return sizeof(PrepInfo); }
+
+void batch_fill_prep_info(void* dst, uint64_t units, uint64_t sunits, uint64_t user_cap,
+                          uint32_t n_attrs, bool mat) {
+  PrepInfo* I = (PrepInfo*)dst;
+  I->units = units;
+  I->sunits = sunits;
+  I->user_cap = user_cap;
+  I->n_attrs = n_attrs;
+  I->mat = mat ? 1u : 0u;
+}
+
 size_t batch_desc_bytes() { return sizeof(BatchQ); }
 
 void batch_fill_desc(void* dst, const Plan& P, const ProbeOut& out, uint64_t unit0,

@@ -25,6 +25,13 @@ hipError_t launch_batch_direct(int L, bool mat, const void* qs, uint32_t nq,
                                uint64_t unit_begin, uint64_t total, unsigned long long* rows,
                                unsigned long long* counts, unsigned long long* csums,
                                const uint64_t* order, hipStream_t s);
+hipError_t launch_batch_place(const void* info, uint32_t nq, const uint64_t* sampled, void* dq,
+                              int32_t* slab, uint64_t slab_words, uint64_t* cap_out,
+                              uint64_t* off_out, uint32_t* admitted, hipStream_t s);
+hipError_t launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s);
+size_t batch_prep_info_bytes();
+void batch_fill_prep_info(void* dst, uint64_t units, uint64_t sunits, uint64_t user_cap,
+                          uint32_t n_attrs, bool mat);
 
 }  // namespace amj
 
@@ -47,6 +54,21 @@ struct amj_batch {
     bool taken = false;
   };
   std::vector<Query> qs;
+  // A captured batch (amj_batch_capture): its chain recorded into a HIP graph over buffers the
+  // batch owns, replayed by amj_batch_replay.
+  struct Captured {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<std::pair<void*, size_t>> bufs;  // device buffers owned by the graph
+    std::vector<uint32_t> order;                 // launch order: query indices
+    int32_t* slab = nullptr;                     // materialised results, placed on the device
+    uint64_t* acc = nullptr;                     // rows, counts, checksums (3 x nq)
+    uint64_t* caps = nullptr;                    // output rows of each query
+    uint64_t* offs = nullptr;                    // its place in the slab (words)
+    uint32_t* admitted = nullptr;                // it fitted the slab
+    uint64_t replays = 0;
+  } cap;
+  bool captured = false;
   bool ran = false;
   double run_ms = 0.0;
   uint64_t budget = 0;  // bytes the batch's results may take (0: what the device has free)
@@ -66,6 +88,10 @@ void amj_batch_destroy(amj_batch_t* b) {
   if (!b) return;
   for (auto& q : b->qs)
     if (b->ran && !q.taken) amj_result_free(b->ctx, &q.res);
+  if (b->cap.exec) (void)hipGraphExecDestroy(b->cap.exec);
+  if (b->cap.graph) (void)hipGraphDestroy(b->cap.graph);
+  if (!b->cap.bufs.empty()) (void)hipStreamSynchronize(b->ctx->stream);
+  for (auto& x : b->cap.bufs) dfree(b->ctx, x.first, x.second);
   delete b;
 }
 
@@ -428,6 +454,10 @@ extern "C" {
 int amj_batch_run(amj_batch_t* b) {
   if (!b) return AMJ_ERR_ARG;
   amj_ctx_t* ctx = b->ctx;
+  if (b->captured) {
+    set_error(ctx, AMJ_ERR_STATE, "a captured batch runs with amj_batch_replay");
+    return AMJ_ERR_STATE;
+  }
   if (b->ran) {
     set_error(ctx, AMJ_ERR_STATE, "the batch has already run");
     return AMJ_ERR_STATE;
@@ -489,6 +519,221 @@ int amj_batch_info(const amj_batch_t* b, uint32_t* queries, double* run_ms,
   if (queries) *queries = (uint32_t)b->qs.size();
   if (run_ms) *run_ms = b->run_ms;
   if (result_bytes) *result_bytes = b->needed;
+  return AMJ_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------
+// Captured batches.
+// ------------------------------------------------------------------------------------------
+namespace {
+
+int cap_alloc(amj_batch_t* b, size_t bytes, void** p) {
+  AMJ_TRY(dmalloc(b->ctx, p, bytes ? bytes : 8));
+  b->cap.bufs.emplace_back(*p, bytes ? bytes : 8);
+  return AMJ_OK;
+}
+
+// The launches of the chain, enqueued on the context's stream (recorded by the caller).
+int enqueue_chain(amj_batch_t* b, const std::vector<uint64_t>& units,
+                  const std::vector<uint64_t>& sunits, const std::vector<uint64_t>& unit0,
+                  const std::vector<uint64_t>& sunit0, uint64_t stotal, void* sdesc, void* ddesc,
+                  void* info, uint64_t* counts, uint64_t* scan_tmp, uint64_t* sampled,
+                  uint64_t slab_words) {
+  amj_ctx_t* ctx = b->ctx;
+  hipStream_t s = ctx->stream;
+  const auto& order = b->cap.order;
+  const uint32_t nq = (uint32_t)order.size();
+  const size_t db = batch_desc_bytes();
+  // 1. size sample: strided units of every query that needs one, one scan, per-query totals
+  if (stotal) {
+    AMJ_HIP_TRY(ctx, launch_fill_u64(counts, stotal + 1, 0, s));
+    for (uint32_t k = 0; k < nq;) {
+      uint32_t e = k;
+      const int L = b->qs[order[k]].plan.n_levels;
+      while (e < nq && b->qs[order[e]].plan.n_levels == L) ++e;
+      const uint64_t gu = sunit0[e - 1] + sunits[e - 1] - sunit0[k];
+      if (gu)
+        AMJ_HIP_TRY(ctx, launch_batch_count(L, (const uint8_t*)sdesc + db * k, e - k, sunit0[k],
+                                            gu, counts, s));
+      k = e;
+    }
+    AMJ_HIP_TRY(ctx, exclusive_scan_u64(counts, counts, stotal + 1, scan_tmp, s));
+    AMJ_HIP_TRY(ctx, launch_batch_totals(sdesc, nq, counts, sampled, s));
+  } else {
+    AMJ_HIP_TRY(ctx, launch_fill_u64(sampled, nq, 0, s));
+  }
+  // 2. output sizes and places, on the device
+  AMJ_HIP_TRY(ctx, launch_batch_place(info, nq, sampled, ddesc, b->cap.slab, slab_words,
+                                      b->cap.caps, b->cap.offs, b->cap.admitted, s));
+  // 3. one direct-output launch per depth and kind
+  AMJ_HIP_TRY(ctx, launch_fill_u64(b->cap.acc, 3ull * nq, 0, s));
+  unsigned long long* rows = (unsigned long long*)b->cap.acc;
+  for (uint32_t k = 0; k < nq;) {
+    uint32_t e = k;
+    const int L = b->qs[order[k]].plan.n_levels;
+    const bool mat = b->qs[order[k]].materialize;
+    while (e < nq && b->qs[order[e]].plan.n_levels == L && b->qs[order[e]].materialize == mat) ++e;
+    const uint64_t gu = unit0[e - 1] + units[e - 1] - unit0[k];
+    AMJ_HIP_TRY(ctx, launch_batch_direct(L, mat, (const uint8_t*)ddesc + db * k, e - k, unit0[k],
+                                         gu, rows + k, rows + nq + k, rows + 2 * nq + k, nullptr,
+                                         s));
+    k = e;
+  }
+  return AMJ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int amj_batch_capture(amj_batch_t* b, uint64_t slab_bytes) {
+  if (!b) return AMJ_ERR_ARG;
+  amj_ctx_t* ctx = b->ctx;
+  if (b->ran || b->captured) {
+    set_error(ctx, AMJ_ERR_STATE, "the batch has already run or been captured");
+    return AMJ_ERR_STATE;
+  }
+  const uint32_t nq = (uint32_t)b->qs.size();
+  if (nq == 0) {
+    set_error(ctx, AMJ_ERR_ARG, "capture of an empty batch");
+    return AMJ_ERR_ARG;
+  }
+  for (const auto& q : b->qs)
+    if (!q.direct) {
+      set_error(ctx, AMJ_ERR_ARG, "a captured batch holds direct-output queries (opts.dro = 1)");
+      return AMJ_ERR_ARG;
+    }
+  if (slab_bytes == 0) slab_bytes = b->budget ? b->budget : (1ull << 30);
+  hipStream_t s = ctx->stream;
+  // launch order: by depth, then kind (materialising first), as the eager direct pass
+  auto& order = b->cap.order;
+  order.resize(nq);
+  for (uint32_t i = 0; i < nq; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+    const int lx = b->qs[x].plan.n_levels, ly = b->qs[y].plan.n_levels;
+    const bool mx = b->qs[x].materialize, my = b->qs[y].materialize;
+    return lx != ly ? lx < ly : (mx != my && mx);
+  });
+  std::vector<uint64_t> units(nq), sunits(nq), unit0(nq), sunit0(nq);
+  uint64_t total = 0, stotal = 0;
+  const size_t db = batch_desc_bytes(), pib = batch_prep_info_bytes();
+  std::vector<uint8_t> sd(db * nq), dd(db * nq), inf(pib * nq);
+  for (uint32_t k = 0; k < nq; ++k) {
+    const auto& q = b->qs[order[k]];
+    units[k] = probe_units(q.plan);
+    sunits[k] = (q.materialize && !q.capacity) ? std::min(units[k], kSampleUnits) : 0;
+    const uint64_t stride = sunits[k] ? units[k] / sunits[k] : 1;
+    unit0[k] = total;
+    sunit0[k] = stotal;
+    total += units[k];
+    stotal += sunits[k];
+    batch_fill_desc(sd.data() + db * k, q.plan, ProbeOut{}, sunit0[k], sunits[k], stride);
+    batch_fill_desc(dd.data() + db * k, q.plan, ProbeOut{}, unit0[k], units[k], 1);
+    batch_fill_prep_info(inf.data() + pib * k, units[k], sunits[k], q.capacity,
+                         (uint32_t)q.plan.n_attrs, q.materialize);
+  }
+  void *sdesc = nullptr, *ddesc = nullptr, *info = nullptr, *counts = nullptr, *tmp = nullptr,
+       *sampled = nullptr, *p = nullptr;
+  AMJ_TRY(cap_alloc(b, sd.size(), &sdesc));
+  AMJ_TRY(cap_alloc(b, dd.size(), &ddesc));
+  AMJ_TRY(cap_alloc(b, inf.size(), &info));
+  AMJ_TRY(cap_alloc(b, (stotal + 1) * 8, &counts));
+  AMJ_TRY(cap_alloc(b, scan_temp_elems(stotal + 1) * 8, &tmp));
+  AMJ_TRY(cap_alloc(b, (size_t)nq * 8, &sampled));
+  AMJ_TRY(cap_alloc(b, (size_t)nq * 24, &p));
+  b->cap.acc = (uint64_t*)p;
+  AMJ_TRY(cap_alloc(b, (size_t)nq * 8, &p));
+  b->cap.caps = (uint64_t*)p;
+  AMJ_TRY(cap_alloc(b, (size_t)nq * 8, &p));
+  b->cap.offs = (uint64_t*)p;
+  AMJ_TRY(cap_alloc(b, (size_t)nq * 4, &p));
+  b->cap.admitted = (uint32_t*)p;
+  const uint64_t slab_words = slab_bytes / 256 * 64;
+  AMJ_TRY(cap_alloc(b, slab_words * 4, &p));
+  b->cap.slab = (int32_t*)p;
+  AMJ_HIP_TRY(ctx, hipMemcpyAsync(sdesc, sd.data(), sd.size(), hipMemcpyHostToDevice, s));
+  AMJ_HIP_TRY(ctx, hipMemcpyAsync(ddesc, dd.data(), dd.size(), hipMemcpyHostToDevice, s));
+  AMJ_HIP_TRY(ctx, hipMemcpyAsync(info, inf.data(), inf.size(), hipMemcpyHostToDevice, s));
+  AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
+  AMJ_HIP_TRY(ctx, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  const int rc = enqueue_chain(b, units, sunits, unit0, sunit0, stotal, sdesc, ddesc, info,
+                               (uint64_t*)counts, (uint64_t*)tmp, (uint64_t*)sampled,
+                               slab_words);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(s, &g);  // ends the capture on every path
+  if (rc != AMJ_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  AMJ_HIP_TRY(ctx, e);
+  b->cap.graph = g;
+  AMJ_HIP_TRY(ctx, hipGraphInstantiate(&b->cap.exec, g, nullptr, nullptr, 0));
+  b->captured = true;
+  return AMJ_OK;
+}
+
+int amj_batch_replay(amj_batch_t* b) {
+  if (!b) return AMJ_ERR_ARG;
+  amj_ctx_t* ctx = b->ctx;
+  if (!b->captured) {
+    set_error(ctx, AMJ_ERR_STATE, "amj_batch_replay needs a captured batch (amj_batch_capture)");
+    return AMJ_ERR_STATE;
+  }
+  hipStream_t s = ctx->stream;
+  // results of the previous replay: views into the slab lapse, exact reruns not taken are freed
+  if (b->ran)
+    for (auto& q : b->qs)
+      if (!q.taken) amj_result_free(ctx, &q.res);
+  const auto& order = b->cap.order;
+  const uint32_t nq = (uint32_t)order.size();
+  std::vector<uint64_t> h(5 * (size_t)nq);
+  std::vector<uint32_t> adm(nq);
+  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[2], s));
+  AMJ_HIP_TRY(ctx, hipGraphLaunch(b->cap.exec, s));
+  AMJ_HIP_TRY(ctx, hipMemcpyAsync(h.data(), b->cap.acc, (size_t)nq * 24, hipMemcpyDeviceToHost, s));
+  AMJ_HIP_TRY(ctx, hipMemcpyAsync(h.data() + 3 * (size_t)nq, b->cap.caps, (size_t)nq * 8,
+                                  hipMemcpyDeviceToHost, s));
+  AMJ_HIP_TRY(ctx, hipMemcpyAsync(h.data() + 4 * (size_t)nq, b->cap.offs, (size_t)nq * 8,
+                                  hipMemcpyDeviceToHost, s));
+  AMJ_HIP_TRY(ctx, hipMemcpyAsync(adm.data(), b->cap.admitted, (size_t)nq * 4,
+                                  hipMemcpyDeviceToHost, s));
+  AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
+  std::vector<uint32_t> redo;
+  for (uint32_t k = 0; k < nq; ++k) {
+    auto& q = b->qs[order[k]];
+    std::memset(&q.res, 0, sizeof(q.res));
+    q.status = AMJ_OK;
+    q.taken = false;
+    const uint64_t cnt = h[nq + k], cs = h[2 * (size_t)nq + k], capk = h[3 * (size_t)nq + k];
+    if (q.materialize && (!adm[k] || cnt > capk)) {  // no room or overran: run again exactly
+      redo.push_back(order[k]);
+      continue;
+    }
+    if (q.materialize && cnt > 0)
+      for (int a = 0; a < q.plan.n_attrs; ++a)
+        q.res.cols[a] = b->cap.slab + h[4 * (size_t)nq + k] + (uint64_t)a * capk;
+    set_result(q, cnt, cs);
+    q.res.reserved[2] = 3;  // a view into the batch's slab: freeing it releases nothing
+    q.res.passes = q.materialize && !q.capacity ? 2 : 1;
+  }
+  b->ran = true;
+  if (!redo.empty()) {
+    const int rc = run_exact(b, redo);
+    if (rc != AMJ_OK) {
+      for (uint32_t i : redo) b->qs[i].status = rc;
+    }
+  }
+  AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[5], s));
+  AMJ_HIP_TRY(ctx, hipEventSynchronize(ctx->ev[5]));
+  b->run_ms = elapsed_ms(ctx->ev[2], ctx->ev[5]);
+  ++b->cap.replays;
+  for (auto& q : b->qs) {
+    q.res.probe_ms = b->run_ms;
+    q.res.probe_kernel_ms = b->run_ms;
+    q.res.probe_kernel_launches = 1;
+  }
   return AMJ_OK;
 }
 

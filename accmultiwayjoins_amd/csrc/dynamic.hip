@@ -272,6 +272,7 @@ int run_dynamic(amj_ctx_t* ctx, Plan& P, const amj_opts_t* o, amj_result_t* res)
           hipSuccess || clock_khz <= 0)
     clock_khz = 100000;
   A.stall_ticks = (o->ws_stall_ms ? (uint64_t)o->ws_stall_ms : kStallMs) * (uint64_t)clock_khz;
+  A.max_claim = o->ws_claim_units;
   if (o->ws) {
     A.split = o->ws_split_threshold ? (uint64_t)(o->ws_split_threshold + 63) / 64 * 64
                                     : kDefaultSplit;
@@ -438,6 +439,7 @@ int run_ooc(amj_ctx_t* ctx, Plan& P, const amj_opts_t* o, amj_result_t* res) {
           hipSuccess || clock_khz <= 0)
     clock_khz = 100000;
   A.stall_ticks = (o->ws_stall_ms ? (uint64_t)o->ws_stall_ms : kStallMs) * (uint64_t)clock_khz;
+  A.max_claim = o->ws_claim_units;
   if (o->ws) {
     A.split = o->ws_split_threshold ? (uint64_t)(o->ws_split_threshold + 63) / 64 * 64
                                     : kDefaultSplit;

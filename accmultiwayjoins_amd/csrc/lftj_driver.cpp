@@ -289,6 +289,7 @@ static int run_lftj_ws(amj_ctx_t* ctx, const LPlan& P, const amj_opts_t* o, amj_
           hipSuccess || clock_khz <= 0)
     clock_khz = 100000;
   A.stall_ticks = (o->ws_stall_ms ? (uint64_t)o->ws_stall_ms : kLftjStallMs) * (uint64_t)clock_khz;
+  A.max_claim = o->ws_claim_units;
   const bool mat = o->materialize != 0;
   const bool hash = o->checksum || !mat;
   const int final_mode = mat ? (hash ? MODE_WRITE_HASH : MODE_WRITE) : MODE_HASH;

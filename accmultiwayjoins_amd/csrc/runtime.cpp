@@ -1,6 +1,7 @@
 // Host runtime of libamj: context, device buffers, relations, hash build driver, query
 // planning and the AMHJ execution driver (count/scan/write and frequent-item paths).
 #include "runtime.h"
+#include "tpt.h"
 
 #include <algorithm>
 #include <cstring>
@@ -286,6 +287,21 @@ void pool_clear(amj_ctx_t* ctx) {
   ctx->pool_bytes = 0;
 }
 
+int arena_drop(amj_ctx_t* ctx) {
+  if (!ctx->arena.base) return AMJ_OK;
+  if (ctx->arena.free_bytes != ctx->arena.size) {
+    set_error(ctx, AMJ_ERR_STATE, "the result arena still holds results");
+    return AMJ_ERR_STATE;
+  }
+  AMJ_HIP_TRY(ctx, hipStreamSynchronize(ctx->stream));
+  dfree(ctx, ctx->arena.base, ctx->arena.size);
+  ctx->arena.base = nullptr;
+  ctx->arena.size = 0;
+  ctx->arena.free.clear();
+  ctx->arena.free_bytes = 0;
+  return AMJ_OK;
+}
+
 int arena_reserve(amj_ctx_t* ctx, uint64_t bytes) {
   if (ctx->arena.base) {
     set_error(ctx, AMJ_ERR_STATE, "the context already has a result arena");
@@ -496,8 +512,51 @@ static int run_plain(amj_ctx_t* ctx, Plan& P, const amj_opts_t* o, amj_result_t*
 
 // Thread-per-tuple baseline: count pass, scan of the per-lane counts, write pass (the
 // count pass also hashes when nothing is materialised).
+// Open-addressing tables of every level (AMJ_ENGINE_OPEN_ADDR), built from the bucketed
+// tables' key arrays: ceilPow2(2n) slots, filled on the device; freed with the holder.
+struct OaHolder {
+  amj_ctx_t* ctx;
+  OaTables t{};
+  std::vector<std::pair<void*, size_t>> bufs;
+  explicit OaHolder(amj_ctx_t* c) : ctx(c) {}
+  ~OaHolder() {
+    if (!bufs.empty()) (void)hipStreamSynchronize(ctx->stream);
+    for (auto& b : bufs) dfree(ctx, b.first, b.second);
+  }
+  int build(const Plan& P) {
+    for (int l = 0; l < P.n_levels; ++l) {
+      const uint64_t n = P.tab[l].n;
+      if (n >= 0xffffffffull || n > (1ull << 30)) {
+        set_error(ctx, AMJ_ERR_ARG, "open addressing: a build table exceeds 2^30 rows");
+        return AMJ_ERR_ARG;
+      }
+      uint64_t cap = 64;
+      while (cap < 2 * n) cap <<= 1;  // load factor at most 1/2 (OA_HT_RATIO, linear.cuh:11)
+      void* p = nullptr;
+      AMJ_TRY(dmalloc(ctx, &p, cap * 8));
+      bufs.emplace_back(p, cap * 8);
+      AMJ_HIP_TRY(ctx, hipMemsetAsync(p, 0xff, cap * 8, ctx->stream));
+      AMJ_HIP_TRY(ctx, launch_oa_insert(P.tab[l].keys, n, (uint64_t*)p, (uint32_t)(cap - 1),
+                                        ctx->stream));
+      t.slots[l] = (const uint64_t*)p;
+      t.mask[l] = (uint32_t)(cap - 1);
+    }
+    return AMJ_OK;
+  }
+};
+
 static int run_tpt(amj_ctx_t* ctx, Plan& P, const amj_opts_t* o, amj_result_t* res) {
   hipStream_t s = ctx->stream;
+  const bool open_addr = o->engine == AMJ_ENGINE_OPEN_ADDR;
+  OaHolder oa(ctx);
+  if (open_addr) {
+    AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[0], s));
+    AMJ_TRY(oa.build(P));
+    AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[1], s));
+    AMJ_HIP_TRY(ctx, hipEventSynchronize(ctx->ev[1]));
+    res->build_ms = elapsed_ms(ctx->ev[0], ctx->ev[1]);
+  }
+  const OaTables* oat = open_addr ? &oa.t : nullptr;
   const uint64_t nb = tpt_blocks(P), lanes = nb * kBlock;
   void *counts = nullptr, *tmp = nullptr, *bsum = nullptr;
   AMJ_TRY(scratch(ctx, SLOT_UNIT_COUNTS, (lanes + 1) * 8, &counts));
@@ -510,7 +569,7 @@ static int run_tpt(amj_ctx_t* ctx, Plan& P, const amj_opts_t* o, amj_result_t* r
   ProbeOut none{};
   AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
   AMJ_HIP_TRY(ctx, launch_tpt(P, false, hash_in_count, (uint64_t*)counts, nullptr, none,
-                              (uint64_t*)bsum, s));
+                              (uint64_t*)bsum, s, oat));
   AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
   AMJ_HIP_TRY(ctx, exclusive_scan_u64((uint64_t*)counts, (uint64_t*)counts, lanes + 1,
                                       (uint64_t*)tmp, s));
@@ -533,7 +592,7 @@ static int run_tpt(amj_ctx_t* ctx, Plan& P, const amj_opts_t* o, amj_result_t* r
     const bool hash = o->checksum != 0;
     AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
     AMJ_HIP_TRY(ctx, launch_tpt(P, true, hash, nullptr, (const uint64_t*)counts, out,
-                                (uint64_t*)bsum, s));
+                                (uint64_t*)bsum, s, oat));
     AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
     if (hash) AMJ_TRY(finish_checksum(ctx, (const uint64_t*)bsum, nb, &res->checksum));
     AMJ_HIP_TRY(ctx, hipEventSynchronize(ctx->ev[4]));
@@ -547,7 +606,7 @@ static int run_tpt(amj_ctx_t* ctx, Plan& P, const amj_opts_t* o, amj_result_t* r
   res->probe_kernel_ms = kernel_ms;
   res->probe_kernel_launches = mat && total > 0 ? 2 : 1;
   res->rows_written = out.cols[0] ? total : 0;
-  res->path = AMJ_PATH_TPT;
+  res->path = open_addr ? AMJ_PATH_OPEN_ADDR : AMJ_PATH_TPT;
   res->passes = mat && total > 0 ? 2 : 1;
   return AMJ_OK;
 }
@@ -560,13 +619,21 @@ int execute_plan(amj_ctx_t* ctx, Plan& plan, const std::vector<uint32_t>& attr_i
   std::memset(res, 0, sizeof(*res));
   res->num_cols = (uint32_t)plan.n_attrs;
   for (int a = 0; a < plan.n_attrs; ++a) res->attr_ids[a] = attr_ids[a];
-  if (o->engine == AMJ_ENGINE_TPT) {
+  if (o->engine == AMJ_ENGINE_TPT || o->engine == AMJ_ENGINE_OPEN_ADDR) {
     if (o->ooc || plan.probe_on_host || o->ws || o->dro || o->fib) {
       set_error(ctx, AMJ_ERR_ARG,
-                "engine AMJ_ENGINE_TPT runs alone: clear ooc, ws, dro and fib");
+                "engines AMJ_ENGINE_TPT and AMJ_ENGINE_OPEN_ADDR run alone: clear ooc, ws, dro "
+                "and fib");
       return AMJ_ERR_ARG;
     }
     return run_tpt(ctx, plan, o, res);
+  }
+  if (o->engine == AMJ_ENGINE_PW) {
+    if (o->ooc || plan.probe_on_host || o->ws || o->dro || o->fib) {
+      set_error(ctx, AMJ_ERR_ARG, "engine AMJ_ENGINE_PW runs alone: clear ooc, ws, dro and fib");
+      return AMJ_ERR_ARG;
+    }
+    return run_pw(ctx, plan, o, res);
   }
   if (o->engine != AMJ_ENGINE_WAVE) {
     set_error(ctx, AMJ_ERR_ARG, "unknown engine " + std::to_string(o->engine));

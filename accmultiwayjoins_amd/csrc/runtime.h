@@ -106,6 +106,8 @@ void pool_clear(amj_ctx_t* ctx);
 // Reserve `bytes` of device memory as the context's result arena (results come only from it
 // afterwards; AMJ_ERR_NOMEM when it is full). Once per context.
 int arena_reserve(amj_ctx_t* ctx, uint64_t bytes);
+// Free an arena that holds no result (AMJ_ERR_STATE otherwise).
+int arena_drop(amj_ctx_t* ctx);
 void arena_release(amj_ctx_t* ctx, void* p, uint64_t bytes);
 // Largest result the arena can hold now (0 without an arena).
 uint64_t arena_largest(const amj_ctx_t* ctx);
@@ -152,6 +154,8 @@ int run_fib(amj_ctx_t* ctx, const Plan& plan, const amj_opts_t* o, amj_result_t*
 // Plan of exec-AMHJ TRI / FOUR over one edge relation and its table (built with sort_vals=1).
 int make_pattern_plan(amj_ctx_t* ctx, const amj_relation* edges, const amj_hash_table* ht,
                       int query, Plan* plan, std::vector<uint32_t>* ids);
+// Pairwise joins with materialised intermediates (opts.engine = AMJ_ENGINE_PW).
+int run_pw(amj_ctx_t* ctx, Plan& P, const amj_opts_t* o, amj_result_t* res);
 // Persistent probe with work sharing (o->ws) and/or direct result output (o->dro).
 int run_dynamic(amj_ctx_t* ctx, Plan& P, const amj_opts_t* o, amj_result_t* res);
 // Out-of-core probe: the probe relation streamed from host memory in slices (o->ooc, or a

@@ -40,7 +40,14 @@ struct amj_server {
   amj_ctx_t* ctx = nullptr;
   uint64_t budget = 0;  // bytes of results that may be held at once
   uint32_t max_batch = 256;
-  int plain = 0;        // 1: one query per launch
+  int plain = 0;        // 1: one query per launch; 2: one query per launch with direct output
+  bool arena_ready = false;  // the result arena is reserved (under gpu)
+  // Batched mode: NORMAL requests whose probe relation has at least tune_rows rows run alone
+  // with the options amj_autotune picks for their class (depth, materialise, probe size class:
+  // one tuning per class, under gpu); 0 = every request goes into the batch.
+  uint64_t tune_rows = 0;
+  std::map<std::vector<uint64_t>, amj_opts_t> tuned_opts;
+  uint64_t tuned_alone = 0;  // under mu
 
   std::mutex mu, gpu;
   std::condition_variable cv_work, cv_done;
@@ -60,12 +67,13 @@ struct amj_server {
     std::vector<uint32_t> ids;   // NORMAL: chain, ids[0] probes; TRI/FOUR: the edge relation
     int materialize;
     double t_submit;
+    double t_start = 0;  // when the launch (plain) or batch that serves it began
   };
   struct Done {
     int status = AMJ_OK;
     std::string err;
     amj_result_t res{};
-    double t_submit = 0, t_done = 0;
+    double t_submit = 0, t_start = 0, t_done = 0;
     uint64_t bytes = 0;
   };
   struct Outcome {
@@ -104,7 +112,7 @@ struct amj_server {
     std::vector<Req> deferred;  // fit the budget, not the room left now: run after releases
     uint64_t deferred_need = 0; // bytes the first deferred request needs
     uint64_t room;
-    uint64_t batches = 0, splits = 0;
+    uint64_t batches = 0, splits = 0, tuned = 0;
   };
 
   static Outcome failed(const Req& q, int rc, const std::string& msg) {
@@ -114,6 +122,7 @@ struct amj_server {
     o.d.err = msg;
     o.d.t_submit = q.t_submit;
     o.d.t_done = now_ms();
+    o.d.t_start = q.t_start ? q.t_start : o.d.t_done;
     return o;
   }
 
@@ -122,6 +131,7 @@ struct amj_server {
     o.ticket = q.ticket;
     o.d.res = res;
     o.d.t_submit = q.t_submit;
+    o.d.t_start = q.t_start;
     o.d.t_done = now_ms();
     o.d.bytes = result_bytes(res);
     c.room -= std::min(c.room, o.d.bytes);
@@ -141,6 +151,25 @@ struct amj_server {
       }
       return;
     }
+    if (tune_rows) {  // large chains alone, with their tuned engine
+      std::vector<Req> small;
+      for (auto& q : reqs) {
+        if (q.query == AMJ_QUERY_NORMAL && rels[q.ids[0]].rel->rows >= tune_rows) {
+          if (!c.deferred.empty()) c.deferred.push_back(q);
+          else run_one(c, q, true);
+        } else {
+          small.push_back(q);
+        }
+      }
+      reqs.swap(small);
+      if (reqs.empty()) return;
+      if (!c.deferred.empty()) {
+        c.deferred.insert(c.deferred.end(), reqs.begin(), reqs.end());
+        return;
+      }
+    }
+    const double t0 = now_ms();
+    for (auto& q : reqs) q.t_start = t0;
     uint64_t need = 0;
     const int rc = run_batch(c, reqs, &need);
     if (rc == AMJ_ERR_NOMEM && reqs.size() > 1) {
@@ -159,18 +188,38 @@ struct amj_server {
     }
   }
 
-  void run_one(Cycle& c, const Req& q) {
+  void run_one(Cycle& c, Req q, bool tuned = false) {
+    q.t_start = now_ms();
     amj_opts_t o;
     amj_default_opts(&o);
     o.materialize = q.materialize;
-    o.hbm_budget_bytes = ctx->allocated + c.room;
+    o.dro = plain == 2 && q.materialize ? 1 : 0;  // direct output where something is written
     amj_result_t res;
-    int rc;
+    int rc = AMJ_OK;
     if (q.query == AMJ_QUERY_NORMAL) {
       std::vector<amj_hash_table_t*> hts;
       for (size_t i = 1; i < q.ids.size(); ++i) hts.push_back(rels[q.ids[i]].ht);
-      rc = amj_execute(ctx, rels[q.ids[0]].rel, hts.data(), (uint32_t)hts.size(), &o, &res);
+      if (tuned) {
+        uint64_t cls = 0;
+        while ((1ull << (2 * cls)) < rels[q.ids[0]].rel->rows && cls < 31) ++cls;
+        const std::vector<uint64_t> key{(uint64_t)hts.size(), (uint64_t)q.materialize, cls};
+        auto it = tuned_opts.find(key);
+        if (it == tuned_opts.end()) {
+          amj_opts_t t = o;
+          rc = amj_autotune(ctx, rels[q.ids[0]].rel, hts.data(), (uint32_t)hts.size(), &t,
+                            nullptr);
+          if (rc == AMJ_OK) it = tuned_opts.emplace(key, t).first;
+        }
+        if (rc == AMJ_OK) o = it->second;
+        o.materialize = q.materialize;
+        o.checksum = 1;
+        ++c.tuned;
+      }
+      o.hbm_budget_bytes = ctx->allocated + c.room;
+      if (rc == AMJ_OK)
+        rc = amj_execute(ctx, rels[q.ids[0]].rel, hts.data(), (uint32_t)hts.size(), &o, &res);
     } else {
+      o.hbm_budget_bytes = ctx->allocated + c.room;
       Plan plan;
       std::vector<uint32_t> ids;
       rc = make_pattern_plan(ctx, rels[q.ids[0]].rel, rels[q.ids[0]].ht, q.query, &plan, &ids);
@@ -272,6 +321,17 @@ struct amj_server {
       {
         std::lock_guard<std::mutex> g(gpu);
         for (auto& r : rel) amj_result_free(ctx, &r);
+        if (!arena_ready && !reqs.empty()) {  // a reservation that failed at creation
+          const int rc = arena_reserve(ctx, budget);
+          if (rc == AMJ_OK) {
+            arena_ready = true;
+          } else {
+            for (auto& q : reqs)
+              c.out.push_back(failed(q, rc, "cannot reserve the result budget: " +
+                                                ctx->last_error));
+            reqs.clear();
+          }
+        }
         run_group(c, reqs);
       }
       const double t1 = now_ms();
@@ -279,6 +339,7 @@ struct amj_server {
       if (!reqs.empty()) busy_ms += t1 - t0;
       batches += c.batches;
       splits += c.splits;
+      tuned_alone += c.tuned;
       for (auto& o : c.out) {
         if (o.d.status == AMJ_OK) {
           in_use += o.d.bytes;
@@ -310,14 +371,21 @@ int amj_server_create(int device, uint64_t hbm_budget_bytes, uint32_t max_batch,
   auto* s = new amj_server();
   s->ctx = ctx;
   s->budget = hbm_budget_bytes ? hbm_budget_bytes : (uint64_t)ctx->props.totalGlobalMem / 2;
+  if (plain < 0 || plain > 2) {
+    delete s;
+    amj_ctx_destroy(ctx);
+    return AMJ_ERR_ARG;
+  }
   // Every result is carved from one reservation of the budget: no device allocation (and no
-  // free, whose memory this device clears before handing it out again) per request.
+  // free, whose memory this device clears before handing it out again, which costs seconds
+  // for tens of GB) per request. Reserved here, outside any request's latency.
   const int rc = arena_reserve(ctx, s->budget);
   if (rc != AMJ_OK) {
     delete s;
     amj_ctx_destroy(ctx);
     return rc;
   }
+  s->arena_ready = true;
   s->max_batch = max_batch ? max_batch : 256;
   s->plain = plain;
   s->worker = std::thread([s] { s->loop(); });
@@ -401,7 +469,67 @@ int amj_server_submit_pattern(amj_server_t* s, uint32_t edge_rel, int query, int
   return submit(s, query, &edge_rel, 1, materialize, ticket);
 }
 
+int amj_server_set_autotune(amj_server_t* s, uint64_t min_probe_rows) {
+  if (!s) return AMJ_ERR_ARG;
+  std::lock_guard<std::mutex> g(s->gpu);
+  s->tune_rows = min_probe_rows;
+  return AMJ_OK;
+}
+
+int amj_server_tuning(amj_server_t* s, uint64_t* tuned_alone, uint64_t* classes) {
+  if (!s) return AMJ_ERR_ARG;
+  std::lock_guard<std::mutex> g(s->gpu);
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (tuned_alone) *tuned_alone = s->tuned_alone;
+  if (classes) *classes = s->tuned_opts.size();
+  return AMJ_OK;
+}
+
+int amj_server_set_budget(amj_server_t* s, uint64_t hbm_budget_bytes) {
+  if (!s || hbm_budget_bytes == 0) return AMJ_ERR_ARG;
+  std::lock_guard<std::mutex> g(s->gpu);
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->next_ticket != 1) {
+    s->last_err = "the budget can only be set before the first request";
+    return AMJ_ERR_STATE;
+  }
+  // the arena is reserved again at the new size now, not inside a request
+  int rc = arena_drop(s->ctx);
+  if (rc == AMJ_OK) {
+    s->arena_ready = false;
+    rc = arena_reserve(s->ctx, hbm_budget_bytes);
+    s->arena_ready = rc == AMJ_OK;
+  }
+  if (rc != AMJ_OK) {
+    s->last_err = s->ctx->last_error;
+    return rc;
+  }
+  s->budget = hbm_budget_bytes;
+  return AMJ_OK;
+}
+
+int amj_server_memory(amj_server_t* s, uint64_t* resident_bytes, uint64_t* budget_bytes,
+                      uint64_t* device_free_bytes) {
+  if (!s) return AMJ_ERR_ARG;
+  std::lock_guard<std::mutex> g(s->gpu);
+  const uint64_t arena = s->arena_ready ? s->ctx->arena.size : 0;
+  if (resident_bytes)
+    *resident_bytes = s->ctx->allocated > arena ? s->ctx->allocated - arena : 0;
+  if (budget_bytes) *budget_bytes = s->budget;
+  if (device_free_bytes) {
+    size_t f = 0, t = 0;
+    if (hipMemGetInfo(&f, &t) != hipSuccess) f = 0;
+    *device_free_bytes = f;
+  }
+  return AMJ_OK;
+}
+
 int amj_server_wait(amj_server_t* s, uint64_t ticket, amj_result_t* res, double* latency_ms) {
+  return amj_server_wait_ex(s, ticket, res, latency_ms, nullptr);
+}
+
+int amj_server_wait_ex(amj_server_t* s, uint64_t ticket, amj_result_t* res, double* latency_ms,
+                       double* queue_ms) {
   if (!s || !res) return AMJ_ERR_ARG;
   std::unique_lock<std::mutex> lk(s->mu);
   if (ticket == 0 || ticket >= s->next_ticket) return AMJ_ERR_ARG;
@@ -409,6 +537,7 @@ int amj_server_wait(amj_server_t* s, uint64_t ticket, amj_result_t* res, double*
   auto it = s->done.find(ticket);
   const int rc = it->second.status;
   if (latency_ms) *latency_ms = it->second.t_done - it->second.t_submit;
+  if (queue_ms) *queue_ms = it->second.t_start - it->second.t_submit;
   if (rc == AMJ_OK) *res = it->second.res;
   else s->last_err = it->second.err;
   s->done.erase(it);  // the result's bytes stay counted until amj_server_release

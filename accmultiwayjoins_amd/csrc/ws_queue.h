@@ -49,6 +49,7 @@ struct WsArgs {
   uint64_t units;        // units to claim
   uint64_t out_cap;      // output rows available
   uint64_t stall_ticks;  // wall-clock ticks without progress before a wave gives up
+  uint64_t max_claim;    // most units claimed at once (0: kMaxClaim)
 };
 
 // Per-wave sharing state.
@@ -159,8 +160,9 @@ __device__ __forceinline__ void ws_serve(const WsArgs& A, RunUnits&& run_units,
       const unsigned long long seen_u = ld_rlx(&A.ctl->next_unit);
       if (seen_u < A.units) {
         const uint64_t waves = (uint64_t)gridDim.x * kWavesPerBlock;
+        const uint64_t most = A.max_claim ? A.max_claim : kMaxClaim;
         uint64_t k = (A.units - seen_u) / (4 * waves);
-        k = k < 1 ? 1 : (k > kMaxClaim ? kMaxClaim : k);
+        k = k < 1 ? 1 : (k > most ? most : k);
         u = add_relaxed(&A.ctl->next_unit, (unsigned long long)k);
         nu = k;
       }

@@ -308,15 +308,275 @@ def four_leg(G, scale):
             "verified": "AMHJ and ALFTJ DFS agree (count + checksum)"}
 
 
-def serving_leg(G, n_queries, max_rows, rate, seed=42, budget=200 << 30):
-    """Mixed stream (60% chains of 3/4/5 relations, 20% TRI, 20% FOUR) over resident relations:
-    chain relations log-uniform in [10^4, max_rows] rows with Zipf(s ~ U[0, 1.2]) build-side
-    keys (key range = the build size, so a chain returns about as many tuples as its probe
-    relation), graphs R-MAT scale 12..16; Poisson arrivals at `rate` queries/s. Served with
-    continuous batching, then the same stream one query per launch (the plain path)."""
-    import random
+def _serve(mode, sources, stream, gaps, budget, fit_budget=False, tune_rows=0):
+    """Serve `stream` through one amj_server: mode "served" (continuous batching), "plain" (one
+    query per launch, count + write) or "plain-dro" (one query per launch, direct output).
+    sources: relations (host arrays or DeviceColumns) registered once; stream: (kind, rel
+    indices, materialize) per request, submitted after the gaps (Poisson arrivals). With
+    fit_budget the result budget is what is left of `budget` and of the device after the
+    resident relations and tables (set before the first request)."""
     import threading
-    from accmultiwayjoins_amd.serving import Server
+    from accmultiwayjoins_amd.serving import DeviceColumns, Server
+    plain = {"served": 0, "plain": 1, "plain-dro": 2}[mode]
+    n = len(stream)
+    # with fit_budget the server starts with a small arena and is given its result budget
+    # once the resident relations and tables are in place
+    with Server(0, hbm_budget_bytes=(1 << 30) if fit_budget else budget, max_batch=256,
+                plain=plain) as srv:
+        t_reg = time.perf_counter()
+        ids = [srv.add_device_relation(c[0], c[1]) if isinstance(c[0], DeviceColumns)
+               else srv.add_relation(c[0], c[1]) for c in sources]
+        reg_s = time.perf_counter() - t_reg
+        mem = srv.memory()
+        if tune_rows and mode == "served":
+            srv.set_autotune(tune_rows)
+        result_budget = budget
+        if fit_budget:
+            result_budget = max(1 << 30, min(budget - mem["resident_bytes"],
+                                             mem["device_free_bytes"] - (8 << 30)))
+            srv.set_budget(result_budget)
+        tickets, lat, qw, outs, errors = [], [], [], [], []
+        done = threading.Event()
+
+        def waiter():
+            try:
+                for i in range(n):
+                    while len(tickets) <= i:
+                        time.sleep(0.0001)
+                    tm = {}
+                    cnt, cs, ms, _ = srv.wait(tickets[i], timing=tm)
+                    lat.append(ms)
+                    qw.append(tm["queue_ms"])
+                    outs.append((cnt, cs))
+            except Exception as e:  # noqa: BLE001 - reported below
+                errors.append(repr(e))
+            done.set()
+
+        th = threading.Thread(target=waiter, daemon=True)
+        th.start()
+        t0 = time.perf_counter()
+        due = t0
+        for (kind, rel_idx, mat), gap in zip(stream, gaps):
+            due += gap
+            d = due - time.perf_counter()
+            if d > 0:
+                time.sleep(d)
+            if kind == "chain":
+                tickets.append(srv.submit([ids[j] for j in rel_idx], materialize=mat))
+            else:
+                tickets.append(srv.submit_pattern(ids[rel_idx[0]], kind, materialize=mat))
+        arrivals_s = time.perf_counter() - t0
+        deadline = time.perf_counter() + 600
+        while not done.wait(timeout=20):  # progress lines while the stream drains
+            log(f"[bench] serving ({mode}): {len(outs)}/{n} answered")
+            if time.perf_counter() > deadline:
+                raise SystemExit("bench: the serving leg did not finish within 600 s")
+        el = time.perf_counter() - t0
+        th.join()
+        if errors:
+            raise SystemExit(f"bench: a served query failed ({mode}): " + errors[0])
+        st = srv.stats()
+        tun = srv.tuning()
+        log(f"[bench] serving ({mode}): {n} queries in {el:.2f} s, {st['batches']} launches "
+            f"or batches, {tun['tuned_alone']} run alone with tuned options")
+    srt = sorted(lat)
+    qs = sorted(qw)
+    pick = lambda a, q: a[min(len(a) - 1, int(len(a) * q))]  # noqa: E731
+    return {"queries_per_s": n / el, "p50_ms": pick(srt, 0.5), "p99_ms": pick(srt, 0.99),
+            "queue_p50_ms": pick(qs, 0.5), "queue_p99_ms": pick(qs, 0.99),
+            "batches": st["batches"], "seconds": el, "arrivals_s": round(arrivals_s, 3),
+            "register_s": round(reg_s, 3),
+            "resident_gb": round(mem["resident_bytes"] / 1e9, 2),
+            "result_budget_gb": round(result_budget / 1e9, 2),
+            "tuned_alone": tun["tuned_alone"], "tuned_classes": tun["classes"],
+            "busy_s": round(st["busy_ms"] / 1e3, 3)}, outs
+
+
+def serving_config5_leg(n_queries, rate, seed=42, budget=200 << 30, scales=(14, 20),
+                        max_rows=50_000_000, tune_rows=4 << 20):
+    """The configured mixed stream: query i (seed 42 + i) is a chain of 3, 4 or 5 relations
+    (60%, in equal thirds), a triangle query (20%) or a 4-clique query (20%). Chain relations
+    have log-uniform sizes in [10^4, max_rows]; each join is foreign-key style: the build side's
+    key column is Zipf(s) over as many keys as the build relation has rows (s ~ U[0, 1.2] per
+    query), the probe side's uniform over the same keys, so a chain returns about as many tuples
+    as its probe relation; outer columns uniform over [0, 2^30). Graph queries run on their own
+    R-MAT graph, scale ~ U{14..20}, edge factor 16. All data is generated on the device. Chains
+    and triangles are materialised, 4-cliques counted and checksummed (at scale 20 their 1.7e10
+    tuples would take 273 GB). Poisson arrivals at `rate`; the same stream is served with
+    continuous batching, one query per launch (count + write) and one query per launch with
+    direct output; every result's count and checksum must agree, and a few small queries are
+    checked against the CPU oracle."""
+    import random
+    from accmultiwayjoins_amd import Context
+    from accmultiwayjoins_amd.serving import DeviceColumns, device_rmat
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ffi as O
+    gctx = Context(0)  # owns the generated device columns
+    sources, stream = [], []
+    mix = {"chain": 0, "TRI": 0, "FOUR": 0}
+    t0 = time.perf_counter()
+    for i in range(n_queries):
+        qs = seed + i
+        rq = random.Random(qs)
+        u = rq.random()
+        if u < 0.6:
+            k = 3 + rq.randrange(3)
+            sizes = [int(10 ** rq.uniform(4, math.log10(max_rows))) for _ in range(k)]
+            z = rq.uniform(0.0, 1.2)
+            idx = []
+            for j, nrow in enumerate(sizes):
+                c = DeviceColumns(gctx, nrow)
+                if j == 0:
+                    c.uniform(0, 0, 1 << 30, qs, 10 * j)
+                else:
+                    c.zipf(0, sizes[j], z, qs, 10 * j)
+                if j == k - 1:
+                    c.uniform(1, 0, 1 << 30, qs, 10 * j + 1)
+                else:
+                    c.uniform(1, 0, sizes[j + 1], qs, 10 * j + 1)
+                sources.append((c, (j, j + 1)))  # R_j(attr j, attr j + 1)
+                idx.append(len(sources) - 1)
+            stream.append(("chain", idx, True))
+            mix["chain"] += 1
+        else:
+            kind = "TRI" if u < 0.8 else "FOUR"
+            sc = rq.randint(scales[0], scales[1])
+            sources.append((device_rmat(gctx, sc, 16, seed=qs), (0, 1)))
+            stream.append((kind, [len(sources) - 1], kind == "TRI"))
+            mix[kind] += 1
+    rg = random.Random(seed)
+    gaps = [rg.expovariate(rate) for _ in range(n_queries)]
+    gen_s = time.perf_counter() - t0
+    rows = sum(c.rows for c, _a in sources)
+    log(f"[bench] serving: {n_queries} queries, {len(sources)} relations, {rows / 1e9:.2f} G "
+        f"rows generated on the device in {gen_s:.1f} s")
+    # a few small queries against the CPU oracle (count + checksum)
+    chains = sorted((sum(sources[j][0].rows for j in idx), q) for q, (kind, idx, _m)
+                    in enumerate(stream) if kind == "chain")
+    oracle_checked = {}
+    for _tot, q in chains[:3]:
+        rels = [([sources[j][0].to_host(0), sources[j][0].to_host(1)], list(sources[j][1]))
+                for j in stream[q][1]]
+        cnt, cs, _, _, _ = O.mhj(rels)
+        oracle_checked[q] = (cnt, cs)
+    results, legs = {}, {}
+    for mode in ("served", "plain", "plain-dro"):
+        legs[mode], results[mode] = _serve(mode, sources, stream, gaps, budget, fit_budget=True,
+                                           tune_rows=tune_rows)
+    if not (results["served"] == results["plain"] == results["plain-dro"]):
+        raise SystemExit("bench: served and plain results differ on the configured stream")
+    for q, want in oracle_checked.items():
+        if tuple(results["served"][q]) != want:
+            raise SystemExit(f"bench: served query {q} differs from the CPU oracle")
+    for c, _a in sources:
+        c.close()
+    gctx.close()
+    return {"served": legs["served"], "plain": legs["plain"], "plain_dro": legs["plain-dro"],
+            "speedup": legs["served"]["queries_per_s"] / legs["plain"]["queries_per_s"],
+            "speedup_vs_plain_dro": (legs["served"]["queries_per_s"] /
+                                     legs["plain-dro"]["queries_per_s"]),
+            "workload": f"{n_queries} queries (seed 42+i): {mix['chain']} chains of 3/4/5 "
+                        f"relations with log-uniform sizes in [1e4, {max_rows:.0e}] rows and "
+                        f"Zipf s ~ U[0, 1.2] (materialised), {mix['TRI']} TRI (materialised) and "
+                        f"{mix['FOUR']} FOUR (count + checksum) on their own R-MAT graphs of "
+                        f"scale {scales[0]}..{scales[1]}; {len(sources)} relations, "
+                        f"{rows / 1e9:.2f} G rows resident, generated on the device; Poisson "
+                        f"arrivals at {rate} queries/s; {budget / 2**30:.0f} GiB HBM budget, of "
+                        f"which what the resident relations and tables leave holds results",
+            "engines": "served: continuous batching (each cycle runs what arrived during the "
+                       "previous one as one batch; one direct-output launch per depth and "
+                       f"kind), chains probing >= {tune_rows} rows alone with the engine options "
+                       "the autotuner picks for their class; plain: one query per launch, count "
+                       "+ scan + write; plain_dro: one query per launch with direct output",
+            "distinct_queries": n_queries,
+            "generation_s": round(gen_s, 2),
+            "verified": "every result (count + checksum) equal in the three runs; "
+                        f"{len(oracle_checked)} smallest chains equal to the CPU oracle"}
+
+
+def graph_leg(G, reps=20, seed=7):
+    """Host overhead of a batch: 64 small direct-output queries (chains of 3..5 relations of
+    10^3..10^5 rows, triangles on R-MAT 10..12) run eagerly with amj_batch_run (a size-sample
+    launch, a host read of the sizes, allocation, the direct-output launches, a host read of
+    the totals) and as a captured HIP graph replayed with amj_batch_replay (one graph launch,
+    one host read). Wall time per batch both ways; every replay's results equal the eager
+    ones."""
+    import random
+    from accmultiwayjoins_amd import Context
+    from accmultiwayjoins_amd.serving import Batch
+    rng = random.Random(seed)
+    ctx = Context(0)
+    keep, qs = [], []
+    for i in range(56):
+        k = 3 + i % 3
+        sizes = [int(10 ** rng.uniform(3, 5)) for _ in range(k)]
+        rels = G.chain_relations(sizes, max(sizes[1:]), seed=seed + i, zipf_s=rng.uniform(0, 1),
+                                 payload_range=1 << 30)
+        R = [ctx.relation(c, ids) for c, ids in rels]
+        H = [ctx.hash_table(r) for r in R[1:]]
+        keep += R + H
+        qs.append(("chain", R[0], H))
+    for i in range(8):
+        src, dst = G.rmat_oriented(10 + i % 3, 16, seed=seed + 100 + i)
+        E = ctx.relation([src, dst], [0, 1])
+        HE = ctx.hash_table(E, 0, sort_vals=True)
+        keep += [E, HE]
+        qs.append(("TRI", E, HE))
+
+    def fill(b):
+        return [b.add(p, h, materialize=True, dro=1) if kind == "chain"
+                else b.add_pattern(p, h, kind, materialize=True, dro=1) for kind, p, h in qs]
+
+    eager_ms, want = [], None
+    for r in range(reps + 1):
+        b = Batch(ctx)
+        ids = fill(b)
+        t0 = time.perf_counter()
+        b.run()
+        el = time.perf_counter() - t0
+        got = [(x.count, x.checksum) for x in (b.result(q) for q in ids)]
+        b.close()
+        if r:
+            eager_ms.append(el * 1e3)
+        want = got if want is None else want
+        if got != want:
+            raise SystemExit("bench: eager batches differ between runs")
+    b = Batch(ctx)
+    ids = fill(b)
+    t0 = time.perf_counter()
+    b.capture(1 << 30)
+    capture_ms = (time.perf_counter() - t0) * 1e3
+    replay_ms = []
+    for r in range(reps + 1):
+        t0 = time.perf_counter()
+        b.replay()
+        el = time.perf_counter() - t0
+        got = [(x.count, x.checksum) for x in (b.result(q) for q in ids)]
+        if got != want:
+            raise SystemExit("bench: a graph replay differs from the eager batch")
+        if r:
+            replay_ms.append(el * 1e3)
+    b.close()
+    for x in keep:
+        x.close()
+    ctx.close()
+    eager_ms.sort()
+    replay_ms.sort()
+    return {"queries_per_batch": len(qs), "eager_ms_per_batch": round(eager_ms[len(eager_ms) // 2], 3),
+            "replay_ms_per_batch": round(replay_ms[len(replay_ms) // 2], 3),
+            "speedup": round(eager_ms[len(eager_ms) // 2] / replay_ms[len(replay_ms) // 2], 2),
+            "capture_ms": round(capture_ms, 3), "reps": reps,
+            "workload": "64 direct-output queries (56 chains of 3..5 relations of 1e3..1e5 rows, "
+                        "8 TRI on R-MAT 10..12), materialised; median wall time per batch "
+                        "(launches, host reads and allocation included)",
+            "verified": "every replay's counts and checksums equal the eager batch's"}
+
+
+def serving_leg(G, n_queries, max_rows, rate, seed=42, budget=200 << 30):
+    """A smaller stream over a few resident relations (24 chains of 3/4/5 relations of 10^4 ..
+    max_rows rows, R-MAT 12..16 graphs, 60% chain / 20% TRI / 20% FOUR, all materialised):
+    launch overheads dominate, which is where batching gains most."""
+    import random
     rng = random.Random(seed)
     chains = []
     for i in range(24):
@@ -327,79 +587,36 @@ def serving_leg(G, n_queries, max_rows, rate, seed=42, budget=200 << 30):
         chains.append(G.chain_relations(sizes, keys, seed=seed + i, zipf_s=rng.uniform(0, 1.2),
                                         payload_range=1 << 30))
     graphs = [G.rmat_oriented(sc, 16, seed=seed + sc) for sc in (12, 13, 14, 15, 16)]
+    sources, cidx, gidx = [], [], []
+    for ch in chains:
+        cidx.append([])
+        for c, a in ch:
+            sources.append((c, a))
+            cidx[-1].append(len(sources) - 1)
+    for sd in graphs:
+        sources.append(([sd[0], sd[1]], (0, 1)))
+        gidx.append(len(sources) - 1)
     stream = []
     for q in range(n_queries):
         u = rng.random()
         if u < 0.6:
-            stream.append(("chain", rng.randrange(len(chains))))
+            stream.append(("chain", cidx[rng.randrange(len(chains))], True))
         else:
-            stream.append(("TRI" if u < 0.8 else "FOUR", rng.randrange(len(graphs))))
+            stream.append(("TRI" if u < 0.8 else "FOUR", [gidx[rng.randrange(len(graphs))]],
+                           True))
     gaps = [rng.expovariate(rate) for _ in range(n_queries)]
-
-    def serve(plain):
-        with Server(0, hbm_budget_bytes=budget, max_batch=256, plain=plain) as srv:
-            cids = [[srv.add_relation(c, a) for c, a in ch] for ch in chains]
-            gids = [srv.add_relation([sd[0], sd[1]], (0, 1)) for sd in graphs]
-            tickets, lat, outs, errors = [], [], [], []
-            done = threading.Event()
-
-            def waiter():
-                try:
-                    for i in range(n_queries):
-                        while len(tickets) <= i:
-                            time.sleep(0.0001)
-                        cnt, cs, ms, _ = srv.wait(tickets[i])
-                        lat.append(ms)
-                        outs.append((cnt, cs))
-                except Exception as e:  # noqa: BLE001 - reported below
-                    errors.append(repr(e))
-                done.set()
-
-            th = threading.Thread(target=waiter, daemon=True)
-            th.start()
-            t0 = time.perf_counter()
-            due = t0
-            for (kind, k), gap in zip(stream, gaps):
-                due += gap
-                d = due - time.perf_counter()
-                if d > 0:
-                    time.sleep(d)
-                if kind == "chain":
-                    tickets.append(srv.submit(cids[k], materialize=True))
-                else:
-                    tickets.append(srv.submit_pattern(gids[k], kind, materialize=True))
-            deadline = time.perf_counter() + 300
-            while not done.wait(timeout=20):  # progress lines while the stream drains
-                log(f"[bench] serving: {len(outs)}/{n_queries} answered")
-                if time.perf_counter() > deadline:
-                    raise SystemExit("bench: the serving leg did not finish within 300 s")
-            el = time.perf_counter() - t0
-            th.join()
-            if errors:
-                raise SystemExit("bench: a served query failed: " + errors[0])
-            st = srv.stats()
-            log(f"[bench] serving ({'plain' if plain else 'batched'}): {n_queries} queries in "
-                f"{el:.2f} s")
-        lat.sort()
-        return {"queries_per_s": n_queries / el, "p50_ms": lat[len(lat) // 2],
-                "p99_ms": lat[min(len(lat) - 1, int(len(lat) * 0.99))],
-                "batches": st["batches"], "seconds": el}, outs
-
-    served, a = serve(False)
-    plain, b = serve(True)
+    served, a = _serve("served", sources, stream, gaps, budget)
+    plain, b = _serve("plain", sources, stream, gaps, budget)
     if a != b:
         raise SystemExit("bench: served and plain results differ")
     return {"served": served, "plain": plain,
             "speedup": served["queries_per_s"] / plain["queries_per_s"],
-            "workload": f"{n_queries} mixed queries (60% chains of 3/4/5 relations of 1e4.."
-                        f"{max_rows} rows, 20% TRI, 20% FOUR on R-MAT 12..16), Poisson "
-                        f"arrivals at {rate} queries/s, results materialised, "
-                        f"{budget / 2**30:.0f} GiB HBM budget for held results",
-            "engines": "served: continuous batching, one direct-output launch per depth per "
-                       "batch (outputs sized from a sample launch for queries not seen before, "
-                       "from the last exact count for repeated ones); plain: one query per "
-                       "launch, count + scan + write",
-            "distinct_queries": len(set(stream)),
+            "workload": f"{n_queries} mixed queries over 24 resident chains (3/4/5 relations of "
+                        f"1e4..{max_rows} rows) and 5 resident R-MAT graphs (scale 12..16), "
+                        f"60% chain / 20% TRI / 20% FOUR, Poisson arrivals at {rate} queries/s, "
+                        f"results materialised, {budget / 2**30:.0f} GiB HBM budget for held "
+                        f"results",
+            "distinct_queries": len(set((k, tuple(i)) for k, i, _m in stream)),
             "verified": "every result (count + checksum) equal in the served and plain runs"}
 
 
@@ -424,8 +641,11 @@ def main():
     ap.add_argument("--no-ablation", action="store_true",
                     help="skip the thread-per-tuple baseline run on the first skew")
     ap.add_argument("--serve-queries", type=int, default=512)
-    ap.add_argument("--serve-max-rows", type=int, default=2_000_000)
-    ap.add_argument("--serve-rate", type=float, default=4000.0)
+    ap.add_argument("--serve-rate", type=float, default=25.0,
+                    help="Poisson arrival rate of the configured stream (queries/s)")
+    ap.add_argument("--serve-max-rows", type=int, default=50_000_000)
+    ap.add_argument("--serve-small", action="store_true",
+                    help="also serve the smaller stream over a few resident relations")
     ap.add_argument("--tri-scale", type=int, default=22)
     ap.add_argument("--ooc-rows", type=int, default=0,
                     help="also run the out-of-core chain leg with this many rows per relation "
@@ -603,10 +823,21 @@ def main():
     if args.four_scale:
         out["four"] = four_leg(G, args.four_scale)
     if not args.no_serving:  # its own server context, after the chain's memory is released
-        serving = serving_leg(G, args.serve_queries, args.serve_max_rows, args.serve_rate)
-        log(f"[bench] serving: {serving['served']['queries_per_s']:.0f} q/s served vs "
-            f"{serving['plain']['queries_per_s']:.0f} q/s plain")
+        serving = serving_config5_leg(args.serve_queries, args.serve_rate,
+                                      max_rows=args.serve_max_rows)
+        log(f"[bench] serving: {serving['served']['queries_per_s']:.1f} q/s served vs "
+            f"{serving['plain']['queries_per_s']:.1f} q/s plain, "
+            f"{serving['plain_dro']['queries_per_s']:.1f} q/s plain with direct output")
         out["serving"] = serving
+    if not args.no_serving:
+        gl = graph_leg(G)
+        log(f"[bench] graph: eager {gl['eager_ms_per_batch']} ms vs replay "
+            f"{gl['replay_ms_per_batch']} ms per batch of {gl['queries_per_batch']}")
+        out["batch_graph"] = gl
+    if args.serve_small:
+        small = serving_leg(G, 512, 2_000_000, 4000.0)
+        log(f"[bench] small stream: {small['speedup']:.2f}x")
+        out["serving_small"] = small
     print(json.dumps(out), flush=True)
 
 

@@ -99,10 +99,20 @@ typedef struct {
                                    AMJ_ERR_STATE instead of hanging (0: 20,000 ms); working
                                    waves report progress as they go, so a slow pass is not
                                    mistaken for a stuck one */
-  uint32_t reserved[3];
+  uint32_t ws_claim_units;      /* ws: most units (64 probe rows or roots each) a wave claims at
+                                   once; claims shrink to a quarter of the remaining units per
+                                   wave (0: 64) */
+  uint32_t reserved[2];
 } amj_opts_t;
 #define AMJ_ENGINE_WAVE 0
 #define AMJ_ENGINE_TPT 1
+#define AMJ_ENGINE_OPEN_ADDR 2   /* the thread-per-tuple walk over per-query open-addressing
+                                    (linear probing) tables: AMHJ_open_addr.cuh with
+                                    Indexing/linear.cuh, for comparison; same results */
+#define AMJ_ENGINE_PW 3          /* pairwise joins with materialised intermediates, each a
+                                    partitioned hash join probed from LDS: PW() of
+                                    HashJoinEPFL/hashJoinEPFL.cuh:174, for comparison; chains
+                                    that bind one new attribute per table */
 
 /* Result of one join. Columns are device memory owned by the library until
  * amj_result_free(); column a holds attribute a (renumbered in first-occurrence order). */
@@ -135,6 +145,8 @@ typedef struct {
 #define AMJ_PATH_OOC 9          /* probe relation streamed in slices (opts.ooc / AMJ_REL_HOST) */
 #define AMJ_PATH_LFTJ_OOC 10    /* triejoin per combination of level-0 value ranges */
 #define AMJ_PATH_TPT 11         /* thread-per-tuple baseline (opts.engine = AMJ_ENGINE_TPT) */
+#define AMJ_PATH_OPEN_ADDR 12   /* open-addressing tables (opts.engine = AMJ_ENGINE_OPEN_ADDR) */
+#define AMJ_PATH_PW 13          /* pairwise partitioned joins (opts.engine = AMJ_ENGINE_PW) */
 
 /* ---- context ------------------------------------------------------------------------ */
 int amj_ctx_create(int device, amj_ctx_t** out);
@@ -263,6 +275,20 @@ int amj_batch_add_pattern(amj_batch_t* batch, const amj_relation_t* edges,
  * amj_batch_run returns AMJ_ERR_NOMEM, with nothing run, when they would take more. */
 int amj_batch_set_budget(amj_batch_t* batch, uint64_t bytes);
 int amj_batch_run(amj_batch_t* batch);
+/* Capture (no counterpart in the reference): record the batch's whole direct-output chain into
+ * a HIP graph. Every query must have been added with opts->dro = 1. Output sizes and places are
+ * decided on the device: a strided size sample of every materialised query, one scan, a
+ * placement kernel that sizes each output (exact when every unit is sampled, else the sample
+ * scaled up by 1.25 + 64 Ki rows) and places it in one result slab of slab_bytes (0: the batch
+ * budget, else 1 GiB) owned by the batch, then one direct-output launch per depth and kind:
+ * no host round trip inside the chain. The relations and tables must outlive the batch.
+ * amj_batch_replay launches the graph and reads the per-query totals once; a materialised
+ * query that did not fit the slab or overran its size runs again exactly (outside the graph).
+ * Results (amj_batch_result) of an admitted query are views into the slab, valid until the
+ * next replay or amj_batch_destroy (amj_result_free on them releases nothing); amj_batch_run
+ * does not apply to a captured batch. Counts and checksums equal amj_batch_run's. */
+int amj_batch_capture(amj_batch_t* batch, uint64_t slab_bytes);
+int amj_batch_replay(amj_batch_t* batch);
 int amj_batch_result(amj_batch_t* batch, uint32_t id, amj_result_t* res);
 /* Queries, device time of the run, and the bytes its materialised results take (also set when
  * amj_batch_run returned AMJ_ERR_NOMEM). */
@@ -279,7 +305,10 @@ int amj_batch_info(const amj_batch_t* batch, uint32_t* queries, double* run_ms,
  * hbm_budget_bytes (bytes of results held by clients until amj_server_release; 0 = half the
  * device; a result counts with its whole output buffer, which direct output sizes from an
  * estimate). plain = 1 serves one query per launch instead (amj_execute's default engine, the
- * comparison path). */
+ * comparison path); plain = 2 one query per launch with direct output (opts.dro = 1), which
+ * separates the gain of batching from that of single-pass output. The result budget is
+ * reserved as one arena at creation; amj_server_set_budget reserves it again at another size
+ * before the first request (e.g. once the resident relations are registered). */
 typedef struct amj_server amj_server_t;
 int amj_server_create(int device, uint64_t hbm_budget_bytes, uint32_t max_batch, int plain,
                       amj_server_t** out);
@@ -295,6 +324,22 @@ int amj_server_submit_pattern(amj_server_t* server, uint32_t edge_rel, int query
                               uint64_t* ticket);
 /* Block until the request is served; latency_ms = submission to completion. */
 int amj_server_wait(amj_server_t* server, uint64_t ticket, amj_result_t* res, double* latency_ms);
+/* The same, with queue_ms = submission to the start of the launch or batch that served it. */
+int amj_server_wait_ex(amj_server_t* server, uint64_t ticket, amj_result_t* res,
+                       double* latency_ms, double* queue_ms);
+/* Batched mode: NORMAL requests whose probe relation has at least min_probe_rows rows run
+ * alone, with the engine options amj_autotune picks once per class of request (depth,
+ * materialise, probe size within a factor of 4); the others go into the batch. 0: off. */
+int amj_server_set_autotune(amj_server_t* server, uint64_t min_probe_rows);
+/* Requests run alone with tuned options, and the classes tuned so far. */
+int amj_server_tuning(amj_server_t* server, uint64_t* tuned_alone, uint64_t* classes);
+/* Set the result budget before the first request (AMJ_ERR_STATE afterwards); the arena is
+ * reserved again at that size then. */
+int amj_server_set_budget(amj_server_t* server, uint64_t hbm_budget_bytes);
+/* Device bytes held for registered relations and their tables, the result budget, and the
+ * device memory free now (any pointer may be NULL). */
+int amj_server_memory(amj_server_t* server, uint64_t* resident_bytes, uint64_t* budget_bytes,
+                      uint64_t* device_free_bytes);
 int amj_server_release(amj_server_t* server, amj_result_t* res);
 int amj_server_copy_col(amj_server_t* server, const amj_result_t* res, uint32_t col,
                         uint64_t row0, uint64_t n, int32_t* host_out);

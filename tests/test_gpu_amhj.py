@@ -19,10 +19,10 @@ PATHS = [dict(), dict(fib=1), dict(fib=1, fib_suffix=1), dict(fib=1, fib_suffix=
          dict(fib=1, probe_order=2),
          dict(ws=1), dict(ws=1, ws_split_threshold=64), dict(dro=1),
          dict(dro=1, dro_capacity_rows=64), dict(ws=1, dro=1, ws_split_threshold=128),
-         dict(engine="tpt")]
+         dict(engine="tpt"), dict(engine="open-addr"), dict(engine="pw")]
 PATH_IDS = ["plain", "fib", "fib-nosuffix", "fib-deepsuffix", "fib-grouped", "ws", "ws-split64",
             "dro",
-            "dro-overflow", "ws-dro", "thread-per-tuple"]
+            "dro-overflow", "ws-dro", "thread-per-tuple", "open-addressing", "pairwise"]
 
 
 def _rels(rec):
@@ -169,11 +169,38 @@ def test_thread_per_tuple_baseline_patterns_and_rules(ctx):
     r4 = ctx.pattern_join(s, d, "FOUR", materialize=False, engine="tpt")
     assert (r4.count, r4.checksum) == (n4, c4) and r4.passes == 1
     rels = G.chain_relations([5000, 3000, 3000], 500, seed=9)
-    with pytest.raises(AmjError, match="runs alone"):
+    with pytest.raises(AmjError, match="runs? alone"):
         ctx.chain_join(rels, engine="tpt", ws=1)
     empty = [(c if i != 1 else [x[:0] for x in c], ids) for i, (c, ids) in enumerate(rels)]
     r = ctx.chain_join(empty, engine="tpt")
     assert (r.count, r.checksum, r.rows_written) == (0, 0, 0)
+
+
+def test_open_addressing_and_pairwise_engines_rules(ctx):
+    # open addressing runs TRI / FOUR (compared columns) to the oracle's results; the pairwise
+    # engine joins chains only and says so; both refuse the wave engines' switches
+    s, d = G.rmat_oriented(10, 16, seed=42)
+    n3, c3 = O.triangles(s, d)
+    n4, c4 = O.four_cliques(s, d)
+    r3 = ctx.pattern_join(s, d, "TRI", fetch=True, engine="open-addr")
+    assert r3.path == "open-addr" and (r3.count, r3.checksum) == (n3, c3)
+    assert O.checksum(r3.tuples()) == c3
+    r4 = ctx.pattern_join(s, d, "FOUR", materialize=False, engine="open-addr")
+    assert (r4.count, r4.checksum) == (n4, c4)
+    with pytest.raises(AmjError, match="pairwise"):
+        ctx.pattern_join(s, d, "TRI", engine="pw")
+    rels = G.chain_relations([5000, 3000, 3000, 2000], 500, seed=9, zipf_s=1.0)
+    want_cnt, want_cs, _, _, _ = O.mhj(rels)
+    r = ctx.chain_join(rels, engine="pw", fetch=True)
+    assert r.path == "pw" and (r.count, r.checksum) == (want_cnt, want_cs)
+    assert O.checksum(r.tuples()) == want_cs
+    for eng in ("open-addr", "pw"):
+        with pytest.raises(AmjError, match="runs? alone"):
+            ctx.chain_join(rels, engine=eng, dro=1)
+    empty = [(c if i != 2 else [x[:0] for x in c], ids) for i, (c, ids) in enumerate(rels)]
+    for eng in ("open-addr", "pw"):
+        r = ctx.chain_join(empty, engine=eng)
+        assert (r.count, r.checksum, r.rows_written) == (0, 0, 0)
 
 
 def test_resident_relations_and_tables_reused(ctx):
