@@ -11,8 +11,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libamj.so")
 GEN_PATH = os.path.join(_HERE, "libamj_gen.so")
 
-AMJ_MAX_ATTRS = 6
-AMJ_MAX_BUILD_TABLES = 5
+AMJ_MAX_ATTRS = 8
+AMJ_MAX_BUILD_TABLES = 7
 
 AMJ_OK = 0
 STATUS = {

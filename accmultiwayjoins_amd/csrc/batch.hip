@@ -42,7 +42,7 @@ __global__ __launch_bounds__(kBlock) void batch_count_kernel(const BatchQ* __res
   const uint64_t g = unit_begin + w;
   const uint32_t q = uniform(query_of(qs, nq, g));
   const BatchQ& Q = qs[q];
-  Lv l0;
+  LvL<L> l0;
   if (Q.stride > 1) {
     // a sample: wave i of the query takes the rows (i * 64 + lane) * stride + stride / 2, so
     // one wave never holds a run of neighbouring rows (edges sorted by source share one hub)
@@ -67,7 +67,7 @@ __global__ __launch_bounds__(kBlock) void batch_write_kernel(const BatchQ* __res
   const uint64_t g = unit_begin + w;
   const uint32_t q = uniform(query_of(qs, nq, g));
   const BatchQ& Q = qs[q];
-  Lv l0;
+  LvL<L> l0;
   make_level0(Q.P, (g - Q.unit0) * kWave, l0);
   Acc acc{};
   acc.out_base = uniform(base[g] - base[Q.unit0]);
@@ -81,7 +81,7 @@ __global__ __launch_bounds__(kBlock) void batch_write_kernel(const BatchQ* __res
 // Direct output (the count pass dropped): every wave gathers its results through the staging
 // buffer of probe_engine.h (DirectStage), one reservation per full buffer on its query's counter.
 template <int L, int t, bool MAT>
-__device__ __forceinline__ void descend_direct(const Plan& P, const Lv& cur, const ProbeOut& out,
+__device__ __forceinline__ void descend_direct(const Plan& P, const LvL<L>& cur, const ProbeOut& out,
                                                Acc& acc, DirectStage& st,
                                                unsigned long long* rows) {
   if constexpr (t == L - 1) {
@@ -89,7 +89,7 @@ __device__ __forceinline__ void descend_direct(const Plan& P, const Lv& cur, con
     acc.count += cur.T;
   } else {
     for (uint64_t c = 0; c < cur.T; c += kWave) {
-      Lv ch;
+      LvL<L> ch;
       make_child<t>(P, cur, c, cur.T, ch);
       if (ch.T) descend_direct<L, t + 1, MAT>(P, ch, out, acc, st, rows);
     }
@@ -107,7 +107,7 @@ __global__ __launch_bounds__(kBlock) void batch_weight_kernel(const BatchQ* __re
   const uint64_t g = unit_begin + w;
   const uint32_t q = uniform(query_of(qs, nq, g));
   const BatchQ& Q = qs[q];
-  Lv l0;
+  LvL<1> l0;  // level 0 binds the probe's attributes (ids 0 and 1) only
   make_level0(Q.P, (g - Q.unit0) * kWave, l0);
   const uint64_t wt = wave_sum_u64((uint64_t)l0.len * (uint64_t)l0.len);
   if (lane_id() == 0) keys[w] = weight_key(wt, w);
@@ -123,15 +123,15 @@ __global__ __launch_bounds__(kBlock) void batch_direct_kernel(
   const uint64_t g = unit_begin + unit_of(order, k);
   const uint32_t q = uniform(query_of(qs, nq, g));
   const BatchQ& Q = qs[q];
-  __shared__ int32_t stage_mem[MAT ? kWavesPerBlock * kMaxAttrs * kDirectRows : 1];
-  DirectStage st{(lds_i32*)&stage_mem[MAT ? (threadIdx.x / kWave) * kMaxAttrs * kDirectRows : 0],
+  __shared__ int32_t stage_mem[MAT ? direct_stage_words<L>() : 1];
+  DirectStage st{(lds_i32*)&stage_mem[MAT ? (threadIdx.x / kWave) * na_of(L) * kDirectRows : 0],
                  0u};
-  Lv l0;
+  LvL<L> l0;
   make_level0(Q.P, (g - Q.unit0) * kWave, l0);
   Acc acc{};
   if (l0.T) {  // MAT: every query of this launch materialises (one instantiation per kind)
     descend_direct<L, 0, MAT>(Q.P, l0, Q.out, acc, st, &rows[q]);
-    if (MAT) direct_flush(Q.out, Q.P.n_attrs, st, &rows[q]);
+    if (MAT) direct_flush<na_of(L)>(Q.out, Q.P.n_attrs, st, &rows[q]);
   }
   const uint64_t cs = wave_sum_u64(acc.csum);
   if (lane_id() == 0) {
@@ -166,7 +166,7 @@ hipError_t launch_batch_count(int L, const void* qs, uint32_t nq, uint64_t unit_
     hipLaunchKernelGGL(batch_count_kernel<LV>, dim3(grid_units(total)), dim3(kBlock), 0, s, \
                        q, nq, unit_begin, total, counts);                                  \
     break;
-    C(1) C(2) C(3) C(4) C(5)
+    C(1) C(2) C(3) C(4) C(5) C(6) C(7)
 #undef C
     default: return hipErrorInvalidValue;
   }
@@ -184,7 +184,7 @@ hipError_t launch_batch_write(int L, const void* qs, uint32_t nq, uint64_t unit_
     hipLaunchKernelGGL(batch_write_kernel<LV>, dim3(grid_units(total)), dim3(kBlock), 0, s, \
                        q, nq, unit_begin, total, base, csums);                             \
     break;
-    C(1) C(2) C(3) C(4) C(5)
+    C(1) C(2) C(3) C(4) C(5) C(6) C(7)
 #undef C
     default: return hipErrorInvalidValue;
   }
@@ -216,7 +216,7 @@ hipError_t launch_batch_direct(int L, bool mat, const void* qs, uint32_t nq,
                          dim3(kBlock), 0, s, q, nq, unit_begin, total, rows, counts, csums,    \
                          order);                                                               \
     break;
-    C(1) C(2) C(3) C(4) C(5)
+    C(1) C(2) C(3) C(4) C(5) C(6) C(7)
 #undef C
     default: return hipErrorInvalidValue;
   }

@@ -43,13 +43,13 @@ constexpr int kTaskLaneWords = kMaxAttrs + 4;
 constexpr int kTaskWords = kTaskHdr + kTaskLaneWords * kWave;
 
 // Publish the level frontier `l` with child range [c0, c1) into slot `slot`.
-template <int t>
-__device__ __forceinline__ void write_task(const DynArgs& A, uint64_t slot, const Lv& l,
+template <int t, int NA>
+__device__ __forceinline__ void write_task(const DynArgs& A, uint64_t slot, const LvN<NA>& l,
                                            uint64_t c0, uint64_t c1) {
   uint32_t* rec = A.tasks + slot * (uint64_t)kTaskWords;
   const int lane = lane_id();
   uint32_t* lw = rec + kTaskHdr;
-  sfor<0, kMaxAttrs>([&](auto a) { lw[a * kWave + lane] = (uint32_t)l.b[a]; });
+  sfor<0, NA>([&](auto a) { lw[a * kWave + lane] = (uint32_t)l.b[a]; });
   lw[(kMaxAttrs + 0) * kWave + lane] = l.lo;
   lw[(kMaxAttrs + 1) * kWave + lane] = l.len;
   lw[(kMaxAttrs + 2) * kWave + lane] = (uint32_t)l.excl;
@@ -66,8 +66,9 @@ __device__ __forceinline__ void write_task(const DynArgs& A, uint64_t slot, cons
   publish_task(A, slot);
 }
 
+template <int NA>
 __device__ __forceinline__ void read_task(const DynArgs& A, uint64_t slot, int& t, uint64_t& c0,
-                                          uint64_t& c1, Lv& l) {
+                                          uint64_t& c1, LvN<NA>& l) {
   const uint32_t* rec = A.tasks + slot * (uint64_t)kTaskWords;
   const int lane = lane_id();
   t = (int)uniform(rec[0]);
@@ -75,14 +76,14 @@ __device__ __forceinline__ void read_task(const DynArgs& A, uint64_t slot, int& 
   c1 = uniform(((uint64_t)rec[4] << 32) | rec[3]);
   l.T = uniform(((uint64_t)rec[6] << 32) | rec[5]);
   const uint32_t* lw = rec + kTaskHdr;
-  sfor<0, kMaxAttrs>([&](auto a) { l.b[a] = (int32_t)lw[a * kWave + lane]; });
+  sfor<0, NA>([&](auto a) { l.b[a] = (int32_t)lw[a * kWave + lane]; });
   l.lo = lw[(kMaxAttrs + 0) * kWave + lane];
   l.len = lw[(kMaxAttrs + 1) * kWave + lane];
   l.excl = ((uint64_t)lw[(kMaxAttrs + 3) * kWave + lane] << 32) | lw[(kMaxAttrs + 2) * kWave + lane];
 }
 
 template <int L, int t, int MODE>
-__device__ __forceinline__ void dyn_descend(const Plan& P, const Lv& cur, uint64_t c0, uint64_t c1,
+__device__ __forceinline__ void dyn_descend(const Plan& P, const LvL<L>& cur, uint64_t c0, uint64_t c1,
                             const ProbeOut& out, const DynArgs& A, Acc& acc, DynWave& W) {
   const auto step = [&]() { ws_step(A, W.q); };
   if constexpr (t == L - 1) {
@@ -116,7 +117,7 @@ __device__ __forceinline__ void dyn_descend(const Plan& P, const Lv& cur, uint64
           c1 = ws_keep_end(c, c1, A.split, k, got);
         }
       }
-      Lv ch;
+      LvL<L> ch;
       make_child<t>(P, cur, c, c1, ch);
       if (ch.T) dyn_descend<L, t + 1, MODE>(P, ch, 0, ch.T, out, A, acc, W);
     }
@@ -127,15 +128,15 @@ template <int L, int MODE>
 __global__ __launch_bounds__(kBlock) void dyn_kernel(const Plan P, const ProbeOut out,
                                                      const DynArgs A) {
   constexpr bool MAT = mode_mat<MODE>();
-  __shared__ int32_t stage_mem[MAT ? kWavesPerBlock * kMaxAttrs * kDirectRows : 1];
-  DynWave W{{(lds_i32*)&stage_mem[MAT ? (threadIdx.x / kWave) * kMaxAttrs * kDirectRows : 0], 0u},
+  __shared__ int32_t stage_mem[MAT ? direct_stage_words<L>() : 1];
+  DynWave W{{(lds_i32*)&stage_mem[MAT ? (threadIdx.x / kWave) * na_of(L) * kDirectRows : 0], 0u},
             {false, 0u}};
   Acc acc{};
   ws_serve(
       A,
       [&](uint64_t u, uint64_t ue) {
         for (uint64_t v = u; v < ue; ++v) {
-          Lv l0;
+          LvL<L> l0;
           make_level0(P, v * kWave, l0);
           if (l0.T) dyn_descend<L, 0, MODE>(P, l0, 0, l0.T, out, A, acc, W);
         }
@@ -143,13 +144,13 @@ __global__ __launch_bounds__(kBlock) void dyn_kernel(const Plan P, const ProbeOu
       [&](uint64_t slot) {
         int t = 0;
         uint64_t c0 = 0, c1 = 0;
-        Lv l;
+        LvL<L> l;
         read_task(A, slot, t, c0, c1, l);
         sfor<0, L - 1>([&](auto tt) {
           if ((int)tt == t) dyn_descend<L, tt, MODE>(P, l, c0, c1, out, A, acc, W);
         });
       });
-  if (MAT) direct_flush(out, P.n_attrs, W.st, &A.ctl->out_rows);
+  if (MAT) direct_flush<na_of(L)>(out, P.n_attrs, W.st, &A.ctl->out_rows);
   ws_accumulate(A, acc.count, acc.csum);
 }
 
@@ -161,7 +162,7 @@ __global__ __launch_bounds__(kBlock) void dyn_sample_kernel(const Plan P, uint64
   const uint64_t s = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
   if (s >= samples) return;  // no barriers in this kernel
   const uint64_t u = s * units / samples;
-  Lv l0;
+  LvL<L> l0;
   make_level0(P, u * kWave, l0);
   Acc acc{};
   const DynArgs none{};  // no sharing, no output
@@ -227,6 +228,8 @@ int dyn_occupancy_grid(const Plan& P, bool mat, bool hash, int cus) {
     case 3: OCC(3); break;
     case 4: OCC(4); break;
     case 5: OCC(5); break;
+    case 6: OCC(6); break;
+    case 7: OCC(7); break;
     default: break;
   }
 #undef OCC

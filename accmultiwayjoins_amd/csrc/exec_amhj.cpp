@@ -118,9 +118,11 @@ int main(int argc, char** argv) {
                      ? amj_chain_join(ctx, views.data(), (uint32_t)views.size(), &o, &res)
                      : amj_pattern_join(ctx, views.data(), query, &o, &res);
   if (rc != AMJ_OK) die("join failed (%d): %s", rc, amj_last_error(ctx));
-  static const char* paths[] = {"?",      "plain", "fib",   "ws",  "dro",
-                                "lftj-dfs", "lftj-bfs", "ws-dro", "batch", "ooc"};
-  log_info("Engine: %s", res.path < 10 ? paths[res.path] : "?");
+  static const char* paths[] = {"?",        "plain",    "fib",    "ws",       "dro",
+                                "lftj-dfs", "lftj-bfs", "ws-dro", "batch",    "ooc",
+                                "lftj-ooc", "tpt",      "open-addr", "pw"};
+  const uint32_t npaths = sizeof(paths) / sizeof(paths[0]);
+  log_info("Engine: %s", res.path < npaths ? paths[res.path] : "?");
   log_info("Total number of outputs: %llu", (unsigned long long)res.count);
   log_info("---------------------------------");
   log_info("Result checking...");

@@ -33,7 +33,7 @@ __global__ __launch_bounds__(kBlock) void probe_count_kernel(const Plan P,
                                                              uint64_t units) {
   const uint64_t unit = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
   if (unit >= units) return;  // whole wave leaves; this kernel has no barriers
-  Lv l0;
+  LvL<L> l0;
   make_level0(P, unit * kWave, l0);
   Acc acc{};
   if (l0.T) descend<L, 0, MODE_COUNT>(P, l0, ProbeOut{}, acc);
@@ -52,7 +52,7 @@ __global__ __launch_bounds__(kBlock) void probe_write_kernel(const Plan P,
   const uint64_t unit = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
   Acc acc{};
   if (unit < units) {
-    Lv l0;
+    LvL<L> l0;
     make_level0(P, unit * kWave, l0);
     acc.out_base = base ? uniform(base[unit]) : 0;
     if (l0.T) descend<L, 0, MODE>(P, l0, out, acc);

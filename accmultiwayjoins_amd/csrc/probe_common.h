@@ -18,14 +18,20 @@ __device__ __forceinline__ void sfor(F&& f) {
   }
 }
 
+// Attributes bound by a query of L build tables: the probe binds at most two and every table
+// at most one more. Per-lane binding arrays of the frontier engines are sized by it, so a
+// shallow query does not pay for the deepest one's registers.
+constexpr int na_of(int L) { return L + 2 < kMaxAttrs ? L + 2 : kMaxAttrs; }
+
 // Attribute access with a wave-uniform runtime attribute index (lowered to v_cndmask chains).
-template <int I = 0>
-__device__ __forceinline__ int32_t get_attr(const int32_t (&b)[kMaxAttrs], int a) {
-  if constexpr (I == kMaxAttrs - 1) return b[I];
+template <int I = 0, int N>
+__device__ __forceinline__ int32_t get_attr(const int32_t (&b)[N], int a) {
+  if constexpr (I == N - 1) return b[I];
   else return a == I ? b[I] : get_attr<I + 1>(b, a);
 }
-__device__ __forceinline__ void set_attr(int32_t (&b)[kMaxAttrs], int a, int32_t v) {
-  sfor<0, kMaxAttrs>([&](auto i) { b[i] = (a == (int)i) ? v : b[i]; });
+template <int N>
+__device__ __forceinline__ void set_attr(int32_t (&b)[N], int a, int32_t v) {
+  sfor<0, N>([&](auto i) { b[i] = (a == (int)i) ? v : b[i]; });
 }
 
 // ------------------------------------------------------------------------------------------
@@ -182,13 +188,16 @@ __device__ __forceinline__ uint32_t parent_of_slots(uint32_t excl, uint32_t len,
   return wave_scan_max_dpp(p);
 }
 
-// Tuple hash over ceil(n_attrs/2) SplitMix64 rounds; attribute pairs are packed in one word.
-__device__ __forceinline__ uint64_t hash_tuple(const int32_t (&v)[kMaxAttrs], int n_attrs) {
+// Tuple hash over ceil(n_attrs/2) SplitMix64 rounds; attribute pairs are packed in one word
+// (n_attrs <= N: the same value whatever the array size).
+template <int N>
+__device__ __forceinline__ uint64_t hash_tuple(const int32_t (&v)[N], int n_attrs) {
   uint64_t h = (uint64_t)n_attrs;
-  sfor<0, kMaxAttrs / 2>([&](auto j) {
+  sfor<0, (N + 1) / 2>([&](auto j) {
     constexpr int i = 2 * (int)j;
     if (i < n_attrs) {
-      const uint64_t hi = (i + 1 < n_attrs) ? (uint64_t)(uint32_t)v[i + 1] : 0ull;
+      uint64_t hi = 0;
+      if constexpr (i + 1 < N) hi = (i + 1 < n_attrs) ? (uint64_t)(uint32_t)v[i + 1] : 0ull;
       h = mix64(h ^ ((uint64_t)(uint32_t)v[i] | (hi << 32)));
     }
   });
@@ -251,6 +260,8 @@ __device__ __forceinline__ void block_sum_store2(uint64_t a, uint64_t b, uint64_
     case 3: MACRO(3); break;              \
     case 4: MACRO(4); break;              \
     case 5: MACRO(5); break;              \
+    case 6: MACRO(6); break;              \
+    case 7: MACRO(7); break;              \
     default: return hipErrorInvalidValue; \
   }
 

@@ -8,14 +8,18 @@
 
 namespace amj {
 
-// Frontier state of one level (per lane) and its wave-uniform total.
-struct Lv {
-  int32_t b[kMaxAttrs];  // bindings of this lane's partial tuple
+// Frontier state of one level (per lane) and its wave-uniform total; NA bindings (na_of(L)
+// for a query of L tables).
+template <int NA>
+struct LvN {
+  int32_t b[NA];         // bindings of this lane's partial tuple
   uint32_t lo;           // run of matches in this level's table
   uint32_t len;
   uint64_t excl;         // exclusive prefix of len over the wave
   uint64_t T;            // total children of the level (uniform)
 };
+template <int L>
+using LvL = LvN<na_of(L)>;
 
 struct Acc {
   uint64_t count;     // results counted (uniform)
@@ -23,7 +27,8 @@ struct Acc {
   uint64_t csum;      // per-lane checksum partial
 };
 
-__device__ __forceinline__ void finish_level(Lv& l) {
+template <int NA>
+__device__ __forceinline__ void finish_level(LvN<NA>& l) {
   if (__ballot(l.len >= (1u << 26)) == 0ull) {  // 64 runs sum below 2^32
     const uint32_t incl = wave_scan_add_dpp(l.len);
     l.excl = incl - l.len;
@@ -36,14 +41,15 @@ __device__ __forceinline__ void finish_level(Lv& l) {
 }
 
 // Parent lane of flattened child g (g in [c, c+64)).
-__device__ __forceinline__ int parent_lane(const Lv& l, uint64_t c,
-                                           uint64_t gg) {
+template <int NA>
+__device__ __forceinline__ int parent_lane(const LvN<NA>& l, uint64_t c, uint64_t gg) {
   if ((l.T >> 32) == 0)
     return (int)parent_of_slots((uint32_t)l.excl, l.len, (uint32_t)c);
   return find_parent(l.excl, gg, l.T);
 }
 
-__device__ __forceinline__ void lookup_level(const Plan& P, int t, Lv& l, bool valid) {
+template <int NA>
+__device__ __forceinline__ void lookup_level(const Plan& P, int t, LvN<NA>& l, bool valid) {
   const int32_t key = get_attr(l.b, P.key_attr[t]);
   const bool check = P.val_mode[t] == VAL_CHECK;
   const int32_t cval = check ? get_attr(l.b, P.val_attr[t]) : 0;
@@ -51,8 +57,10 @@ __device__ __forceinline__ void lookup_level(const Plan& P, int t, Lv& l, bool v
 }
 
 // Level 0 from probe row r on every lane where `valid`.
-__device__ __forceinline__ void make_level0_at(const Plan& P, uint64_t r, bool valid, Lv& l) {
-  sfor<0, kMaxAttrs>([&](auto a) { l.b[a] = 0; });
+template <int NA>
+__device__ __forceinline__ void make_level0_at(const Plan& P, uint64_t r, bool valid,
+                                               LvN<NA>& l) {
+  sfor<0, NA>([&](auto a) { l.b[a] = 0; });
   for (int c = 0; c < P.probe_ncols; ++c)
     set_attr(l.b, P.probe_attr[c], valid ? P.probe_cols[c][r] : 0);
   lookup_level(P, 0, l, valid);
@@ -60,15 +68,16 @@ __device__ __forceinline__ void make_level0_at(const Plan& P, uint64_t r, bool v
 }
 
 // Level 0 of probe rows [row0, row0 + 64).
-__device__ __forceinline__ void make_level0(const Plan& P, uint64_t row0, Lv& l) {
+template <int NA>
+__device__ __forceinline__ void make_level0(const Plan& P, uint64_t row0, LvN<NA>& l) {
   const uint64_t r = row0 + (uint64_t)lane_id();
   make_level0_at(P, r, r < P.probe_n, l);
 }
 
 // Children [c, min(c+64, end)) of level t become the level-(t+1) frontier (end <= par.T).
-template <int t>
-__device__ __forceinline__ void make_child(const Plan& P, const Lv& par,
-                                           uint64_t c, uint64_t end, Lv& ch) {
+template <int t, int NA>
+__device__ __forceinline__ void make_child(const Plan& P, const LvN<NA>& par, uint64_t c,
+                                           uint64_t end, LvN<NA>& ch) {
   const uint64_t g = c + (uint64_t)lane_id();
   const bool valid = g < end;
   const uint64_t gg = valid ? g : par.T - 1;
@@ -76,7 +85,7 @@ __device__ __forceinline__ void make_child(const Plan& P, const Lv& par,
   const uint32_t plo = __shfl(par.lo, p, kWave);
   const uint32_t pex = (uint32_t)__shfl((uint32_t)par.excl, p, kWave);
   const uint32_t row = plo + ((uint32_t)gg - pex);
-  sfor<0, kMaxAttrs>([&](auto a) { ch.b[a] = (a < P.n_attrs) ? __shfl(par.b[a], p, kWave) : 0; });
+  sfor<0, NA>([&](auto a) { ch.b[a] = (a < P.n_attrs) ? __shfl(par.b[a], p, kWave) : 0; });
   if (P.val_mode[t] == VAL_BIND) set_attr(ch.b, P.val_attr[t], P.tab[t].vals[row]);
   lookup_level(P, t + 1, ch, valid);
   finish_level(ch);
@@ -88,8 +97,8 @@ struct NoStep {
 };
 
 // Emit the results [0, T) of the last level at absolute rows out_row0 + e.
-template <bool MATERIALIZE, bool HASH, typename Step = NoStep>
-__device__ __forceinline__ void emit(const Plan& P, int t, const Lv& l,
+template <bool MATERIALIZE, bool HASH, typename Step = NoStep, int NA>
+__device__ __forceinline__ void emit(const Plan& P, int t, const LvN<NA>& l,
                                      uint64_t out_row0, const ProbeOut& out, Acc& acc,
                                      const Step& step = Step{}) {
   const bool bind = P.val_mode[t] == VAL_BIND;
@@ -105,15 +114,15 @@ __device__ __forceinline__ void emit(const Plan& P, int t, const Lv& l,
     const uint32_t plo = __shfl(l.lo, p, kWave);
     const uint32_t pex = (uint32_t)__shfl((uint32_t)l.excl, p, kWave);
     const uint32_t row = plo + ((uint32_t)gg - pex);
-    int32_t v[kMaxAttrs];
-    sfor<0, kMaxAttrs>([&](auto a) { v[a] = (a < P.n_attrs) ? __shfl(l.b[a], p, kWave) : 0; });
+    int32_t v[NA];
+    sfor<0, NA>([&](auto a) { v[a] = (a < P.n_attrs) ? __shfl(l.b[a], p, kWave) : 0; });
     if (bind) set_attr(v, vattr, vals[row]);
     if (valid) {
       if (HASH) acc.csum += hash_tuple(v, P.n_attrs);
       if (MATERIALIZE) {
         const uint64_t pos = out_row0 + g;
         if (pos < out.capacity) {
-          sfor<0, kMaxAttrs>([&](auto a) {
+          sfor<0, NA>([&](auto a) {
             if (a < P.n_attrs) out.cols[a][pos] = v[a];
           });
         }
@@ -136,6 +145,11 @@ struct DirectStage {
   uint32_t fill;  // rows held (uniform)
 };
 
+// LDS of a block's staging buffers for a query of L tables.
+template <int L>
+constexpr int direct_stage_words() { return kWavesPerBlock * na_of(L) * kDirectRows; }
+
+template <int NA>
 __device__ __forceinline__ void direct_flush(const ProbeOut& out, int na, DirectStage& st,
                                              unsigned long long* rows) {
   if (st.fill == 0) return;
@@ -148,7 +162,7 @@ __device__ __forceinline__ void direct_flush(const ProbeOut& out, int na, Direct
   for (uint32_t i = (uint32_t)lane_id(); i < st.fill; i += kWave) {
     const uint64_t pos = base + i;
     if (pos < out.capacity)
-      sfor<0, kMaxAttrs>([&](auto a) {
+      sfor<0, NA>([&](auto a) {
         if (a < na) out.cols[a][pos] = st.buf[a * kDirectRows + i];
       });
   }
@@ -157,8 +171,8 @@ __device__ __forceinline__ void direct_flush(const ProbeOut& out, int na, Direct
 }
 
 // Results [0, T) of a last level into the wave's buffer (MAT) and the checksum (HASH).
-template <bool MAT, bool HASH, typename Step = NoStep>
-__device__ __forceinline__ void direct_emit(const Plan& P, int t, const Lv& l,
+template <bool MAT, bool HASH, typename Step = NoStep, int NA>
+__device__ __forceinline__ void direct_emit(const Plan& P, int t, const LvN<NA>& l,
                                             const ProbeOut& out, Acc& acc, DirectStage& st,
                                             unsigned long long* rows,
                                             const Step& step = Step{}) {
@@ -175,16 +189,16 @@ __device__ __forceinline__ void direct_emit(const Plan& P, int t, const Lv& l,
     const uint32_t plo = __shfl(l.lo, p, kWave);
     const uint32_t pex = (uint32_t)__shfl((uint32_t)l.excl, p, kWave);
     const uint32_t row = plo + ((uint32_t)gg - pex);
-    int32_t v[kMaxAttrs];
-    sfor<0, kMaxAttrs>([&](auto a) { v[a] = (a < P.n_attrs) ? __shfl(l.b[a], p, kWave) : 0; });
+    int32_t v[NA];
+    sfor<0, NA>([&](auto a) { v[a] = (a < P.n_attrs) ? __shfl(l.b[a], p, kWave) : 0; });
     if (bind) set_attr(v, vattr, vals[row]);
     if (HASH && valid) acc.csum += hash_tuple(v, P.n_attrs);
     if (MAT) {
       const uint32_t m = e1 - c < (uint64_t)kWave ? (uint32_t)(e1 - c) : (uint32_t)kWave;
-      if (st.fill + m > kDirectRows) direct_flush(out, P.n_attrs, st, rows);
+      if (st.fill + m > kDirectRows) direct_flush<NA>(out, P.n_attrs, st, rows);
       __asm__ __volatile__("" ::: "memory");
       if (valid)
-        sfor<0, kMaxAttrs>([&](auto a) {
+        sfor<0, NA>([&](auto a) {
           if (a < P.n_attrs) st.buf[a * kDirectRows + st.fill + lane_id()] = v[a];
         });
       __asm__ __volatile__("" ::: "memory");
@@ -195,7 +209,7 @@ __device__ __forceinline__ void direct_emit(const Plan& P, int t, const Lv& l,
 
 // Depth-first over the frontier levels of one unit (count, or write at acc.out_base).
 template <int L, int t, int MODE>
-__device__ __forceinline__ void descend(const Plan& P, const Lv& cur,
+__device__ __forceinline__ void descend(const Plan& P, const LvL<L>& cur,
                                         const ProbeOut& out, Acc& acc) {
   if constexpr (t == L - 1) {
     if constexpr (MODE == MODE_COUNT) {
@@ -207,7 +221,7 @@ __device__ __forceinline__ void descend(const Plan& P, const Lv& cur,
     }
   } else {
     for (uint64_t c = 0; c < cur.T; c += kWave) {
-      Lv ch;
+      LvL<L> ch;
       make_child<t>(P, cur, c, cur.T, ch);
       if (ch.T) descend<L, t + 1, MODE>(P, ch, out, acc);
     }

@@ -171,6 +171,8 @@ hipError_t launch_tpt(const Plan& P, bool write, bool hash, uint64_t* counts,
     case 3: LT(3); break;
     case 4: LT(4); break;
     case 5: LT(5); break;
+    case 6: LT(6); break;
+    case 7: LT(7); break;
     default: return hipErrorInvalidValue;
   }
 #undef LT

@@ -106,6 +106,8 @@ hipError_t launch_window_write(const Plan& P, const uint64_t* probe_off, uint64_
     case 3: launch_window_l3(PC, NV, mode, grid, s, P, a); break;
     case 4: launch_window_l4(PC, NV, mode, grid, s, P, a); break;
     case 5: launch_window_l5(PC, NV, mode, grid, s, P, a); break;
+    case 6: launch_window_l6(PC, NV, mode, grid, s, P, a); break;
+    case 7: launch_window_l7(PC, NV, mode, grid, s, P, a); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

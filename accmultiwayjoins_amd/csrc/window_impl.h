@@ -360,6 +360,8 @@ void launch_chain_nv(int nv, int mode, dim3 grid, hipStream_t s, const Plan& P,
     case 3: launch_chain<L, PC, 3>(mode, grid, s, P, a); break;
     case 4: launch_chain<L, PC, 4>(mode, grid, s, P, a); break;
     case 5: launch_chain<L, PC, 5>(mode, grid, s, P, a); break;
+    case 6: launch_chain<L, PC, 6>(mode, grid, s, P, a); break;
+    case 7: launch_chain<L, PC, 7>(mode, grid, s, P, a); break;
     default: break;
   }
 }
@@ -374,6 +376,10 @@ void launch_window_l3(int pc, int nv, int mode, dim3 grid, hipStream_t s, const 
 void launch_window_l4(int pc, int nv, int mode, dim3 grid, hipStream_t s, const Plan& P,
                       const WindowArgs& a);
 void launch_window_l5(int pc, int nv, int mode, dim3 grid, hipStream_t s, const Plan& P,
+                      const WindowArgs& a);
+void launch_window_l6(int pc, int nv, int mode, dim3 grid, hipStream_t s, const Plan& P,
+                      const WindowArgs& a);
+void launch_window_l7(int pc, int nv, int mode, dim3 grid, hipStream_t s, const Plan& P,
                       const WindowArgs& a);
 
 #define AMJ_DEFINE_WINDOW_L(LV)                                                                \

@@ -204,24 +204,49 @@ def triangle_leg(ctx, G, scale, steps, warmup):
                          "verified_against": "ALFTJ DFS (count + checksum)"}}
 
 
-def ablation_leg(ctx, resident, s, checked, wave_probe_ms):
-    """The thread-per-tuple baseline engine (one lane per probe row, no cooperation in the
-    wave) on the same resident chain at skew s, materialising, against the wave engine's
-    probe time from the timed steps; its count and checksum must equal the verified ones."""
+def ablation_leg(ctx, resident, s, checked, wave_probe_ms, G):
+    """The comparison engines on the same resident chain at skew s, materialising, against the
+    wave engine's probe time from the timed steps (counts and checksums must equal the verified
+    ones): thread per tuple (one lane per probe row, MHJ.cuh) and the same walk over per-query
+    open-addressing tables (AMHJ_open_addr.cuh; their build reported apart). The pairwise
+    partitioned join (PW, hashJoinEPFL.cuh:174) materialises intermediates of up to 2^32 rows,
+    so it runs the chain at a quarter of the size, beside the wave engine on that chain."""
     R = resident[s]
     H = [ctx.hash_table(r) for r in R[1:]]
-    r = ctx.execute(R[0], H, engine="tpt", materialize=True, checksum=True)
-    ctx.free(r)
+    out = {"skew": s, "wave_probe_ms": wave_probe_ms}
+    for eng, key in (("tpt", "thread_per_tuple"), ("open-addr", "open_addressing")):
+        r = ctx.execute(R[0], H, engine=eng, materialize=True, checksum=True)
+        ctx.free(r)
+        if (r.count, r.checksum) != checked[:2]:
+            raise SystemExit(f"bench: {eng} result differs at s={s}")
+        out[key + "_probe_ms"] = round(r.probe_ms, 3)
+        if eng == "open-addr":
+            out["open_addressing_build_ms"] = round(r.build_ms, 3)
+        log(f"[bench] ablation s={s}: {eng} probe {r.probe_ms:.1f} ms vs wave engine "
+            f"{wave_probe_ms:.1f} ms")
     for h in H:
         h.close()
-    if (r.count, r.checksum) != checked[:2]:
-        raise SystemExit(f"bench: thread-per-tuple result differs at s={s}")
-    log(f"[bench] ablation s={s}: thread-per-tuple probe {r.probe_ms:.1f} ms vs wave engine "
-        f"{wave_probe_ms:.1f} ms")
-    return {"skew": s, "thread_per_tuple_probe_ms": round(r.probe_ms, 3),
-            "wave_probe_ms": wave_probe_ms,
-            "speedup": round(r.probe_ms / wave_probe_ms, 2) if wave_probe_ms > 0 else None,
-            "verified": True}
+    out["speedup"] = round(out["thread_per_tuple_probe_ms"] / wave_probe_ms, 2)
+    out["speedup_vs_open_addressing"] = round(out["open_addressing_probe_ms"] / wave_probe_ms, 2)
+    # pairwise, on the chain at a quarter of the size (R1 16 Mi rows, R2..R4 4 Mi, 1 Mi keys)
+    q = make_inputs(G, 0.25, s)
+    Rq = [ctx.relation(c, ids) for c, ids in q]
+    Hq = [ctx.hash_table(r) for r in Rq[1:]]
+    w = ctx.execute(Rq[0], Hq, materialize=True, checksum=True, ws=1, dro=1, fib=1)
+    ctx.free(w)
+    pw = ctx.execute(Rq[0], Hq, engine="pw", materialize=True, checksum=True)
+    ctx.free(pw)
+    for x in (*Hq, *Rq):
+        x.close()
+    if (pw.count, pw.checksum) != (w.count, w.checksum):
+        raise SystemExit(f"bench: pairwise result differs at s={s}")
+    log(f"[bench] ablation s={s}, quarter chain: pairwise {pw.probe_ms:.1f} ms vs wave engine "
+        f"{w.probe_ms:.1f} ms ({pw.count} tuples)")
+    out["pairwise"] = {"chain": "R1 16 Mi rows, R2..R4 4 Mi rows, 1 Mi keys", "count": pw.count,
+                       "pairwise_ms": round(pw.probe_ms, 3), "wave_probe_ms": round(w.probe_ms, 3),
+                       "speedup": round(pw.probe_ms / w.probe_ms, 2)}
+    out["verified"] = True
+    return out
 
 
 def ooc_leg(G, rows, budget_gb):
@@ -758,7 +783,7 @@ def main():
     ablation = None
     if not args.no_ablation:
         ablation = ablation_leg(ctx, resident, skews[0], checked[skews[0]],
-                                per_skew[str(skews[0])]["probe_ms"])
+                                per_skew[str(skews[0])]["probe_ms"], G)
 
     tri = None
     if not args.no_triangle:

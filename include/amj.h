@@ -32,8 +32,11 @@
 extern "C" {
 #endif
 
-#define AMJ_MAX_BUILD_TABLES 5   /* MHJ-GPU/conf.h:18 MAX_NUM_BUILDTABLES */
-#define AMJ_MAX_ATTRS 6          /* MHJ-GPU/conf.h:20 MAX_NUM_RES_ATTRS */
+#define AMJ_MAX_BUILD_TABLES 7   /* MHJ-GPU/conf.h:16-18 MAX_NUM_BUILDTABLES, at the value the
+                                    reference sets for TPC-H Q8 */
+#define AMJ_MAX_ATTRS 8          /* MHJ-GPU/conf.h:16-20 MAX_NUM_RES_ATTRS, likewise (Q8) */
+/* Row limits: a build table holds fewer than 2^32 rows (bucket pointers and row indexes are
+ * 32-bit), and so does a trie level; probe relations and results are counted in 64 bits. */
 #define AMJ_MAX_PROBE_COLS 2     /* MHJ-GPU/conf.h:19 MAX_NUM_ATTRS_IN_BUILD_TABLE */
 
 typedef enum {

@@ -215,10 +215,29 @@ def test_resident_relations_and_tables_reused(ctx):
 
 def test_too_many_attributes_rejected(ctx):
     rng = np.random.default_rng(1)
-    rels = _random_chain(rng, 6, 50, keys=5)  # 7 attributes > AMJ_MAX_ATTRS
+    rels = _random_chain(rng, 8, 50, keys=5)  # 9 attributes > AMJ_MAX_ATTRS (8)
     with pytest.raises(AmjError) as e:
         ctx.chain_join(rels)
     assert "too many attributes" in str(e.value)
+
+
+@pytest.mark.parametrize("n_rel", [6, 7, 8])
+@pytest.mark.parametrize("opts", [dict(), dict(fib=1), dict(ws=1, dro=1, ws_split_threshold=64),
+                                  dict(engine="tpt"), dict(engine="pw")],
+                         ids=["plain", "fib", "ws-dro", "thread-per-tuple", "pairwise"])
+def test_deep_chains_up_to_seven_build_tables(ctx, n_rel, opts):
+    # the reference's TPC-H Q8 setting (MHJ-GPU/conf.h:16-20): 7 build tables, 8 attributes
+    rels = G.chain_relations([3000] + [2000] * (n_rel - 1), 1500, seed=50 + n_rel, zipf_s=0.6)
+    if n_rel == 8:  # 7 build tables within 8 attributes: the last table holds its key only
+        cols, ids = rels[-1]
+        rels[-1] = (cols[:1], ids[:1])
+    n_attrs = len({a for _, ids in rels for a in ids})
+    assert n_attrs <= 8
+    want_cnt, want_cs, _, _, _ = O.mhj(rels)
+    r = ctx.chain_join(rels, fetch=True, **opts)
+    assert (r.count, r.checksum) == (want_cnt, want_cs) and r.count > 0
+    assert len(r.cols) == n_attrs
+    assert O.checksum(r.tuples()) == want_cs
 
 
 def test_errors_are_reported_not_fatal(ctx):
