@@ -15,9 +15,9 @@ struct LRel {
   const int32_t* d0;
   const uint32_t* off;  // [n0 + 1]
   const int32_t* d1;
-  const uint32_t* dense;  // optional: dense[v] = row of v in d0 (0xffffffff: absent)
+  const uint2* vrange;  // optional: vrange[v] = child range [x, y) of value v (empty: absent)
   uint32_t n0;
-  uint32_t dense_n;
+  uint32_t vrange_n;
 };
 
 struct TrieDev {
@@ -25,8 +25,8 @@ struct TrieDev {
   int32_t* d0 = nullptr;
   uint32_t* off = nullptr;
   int32_t* d1 = nullptr;
-  uint32_t* dense = nullptr;
-  uint32_t n0 = 0, n1 = 0, dense_n = 0;
+  uint2* vrange = nullptr;
+  uint32_t n0 = 0, n1 = 0, vrange_n = 0;
   uint32_t max_run = 0;  // longest child list
   double build_ms = 0.0;
 };
@@ -42,9 +42,27 @@ struct LPlan {
   uint32_t root_mask[kMaxLftjAttrs];   // relations whose first attribute is i
   uint32_t child_mask[kMaxLftjAttrs];  // relations whose second attribute is i
   int32_t driver0;                     // root relation whose level 0 enumerates attribute 0
-  int32_t pad;
+  int32_t off32;                       // every trie array is addressable with 32-bit byte offsets
   uint64_t roots;                      // level-0 size of driver0
 };
+
+// Query shapes with kernels of their own: relations listed by (x, y) ascending (the order the
+// plan puts them in). The code has, for attribute i, bit 12i + r set when relation r is rooted
+// at i and bit 12i + 6 + r when i is its child attribute.
+constexpr uint64_t lftj_shape_code(const int (*xy)[2], int n) {
+  uint64_t c = 0;
+  for (int r = 0; r < n; ++r) {
+    c |= 1ull << (12 * xy[r][0] + r);
+    c |= 1ull << (12 * xy[r][1] + 6 + r);
+  }
+  return c;
+}
+constexpr int kTriXY[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+constexpr int kFourXY[6][2] = {{0, 1}, {0, 2}, {0, 3}, {1, 2}, {1, 3}, {2, 3}};
+constexpr uint64_t kShapeTri = lftj_shape_code(kTriXY, 3);    // triangle
+constexpr uint64_t kShapeFour = lftj_shape_code(kFourXY, 6);  // 4-clique
+// The plan's compiled shape (kShapeTri, kShapeFour), or 0: kernels read the masks at run time.
+uint64_t lftj_static_shape(const LPlan& P);
 
 int build_trie(amj_ctx_t* ctx, const int32_t* c0, const int32_t* c1, uint64_t n, TrieDev* out);
 int trie_from_host(amj_ctx_t* ctx, uint64_t n0, const int32_t* d0, const uint64_t* off,
@@ -73,11 +91,19 @@ hipError_t launch_lftj_write(const LPlan& P, const uint64_t* unit_base, const LO
 hipError_t launch_lftj_sample(const LPlan& P, uint64_t* counts, uint64_t waves,
                               uint32_t per_wave, uint64_t stride, uint64_t offset,
                               hipStream_t s);
-// Direct output: one pass, rows reserved with atomics into *rows (zeroed by the caller); rows at
-// or past `capacity` are dropped, so *rows > capacity means the pass must run again.
-hipError_t launch_lftj_direct(const LPlan& P, const LOut& cols, uint64_t capacity, bool hash,
-                              unsigned long long* rows, uint64_t* block_sums, uint64_t* n_blocks,
-                              const uint64_t* order, hipStream_t s);
+// Direct output: one pass, rows reserved `chunk` at a time (a multiple of 256) with atomics
+// into *rows (zeroed by the caller); rows at or past `capacity` are dropped, so
+// *rows > capacity means the pass must run again. tails[capacity / chunk] (zeroed by the
+// caller) receives the unused tail of every wave's last chunk, block_counts the per-block result
+// counts; when *rows <= capacity, launch_chunk_fill (with the exclusive scan of the tails of the
+// *rows / chunk reserved chunks) makes the first n = sum(block_counts) rows the results.
+hipError_t launch_lftj_direct(const LPlan& P, const LOut& cols, uint64_t capacity,
+                              uint32_t chunk, bool hash, unsigned long long* rows,
+                              uint64_t* tails, uint64_t* block_sums, uint64_t* block_counts,
+                              uint64_t* n_blocks, const uint64_t* order, hipStream_t s);
+hipError_t launch_chunk_fill(const LOut& cols, int n_attrs, const uint64_t* tail,
+                             const uint64_t* tpre, uint64_t n_chunks, uint32_t chunk, uint64_t n,
+                             uint64_t reserved, hipStream_t s);
 
 // Work sharing (lftj_ws.hip): persistent waves over the units (taken in `order`, nullptr: number
 // order) that split long candidate runs into tasks. `args` is a WsArgs (ws_queue.h) whose task

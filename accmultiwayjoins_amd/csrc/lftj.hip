@@ -12,33 +12,58 @@
 // attribute is bound, the range of its children; the candidates for the next attribute are the
 // entries of each lane's *shortest* participating list, flattened over the wave and enumerated
 // 64 at a time, so lanes stay busy whatever the degree skew. Each candidate is checked against
-// every other participating list (binary search in a child range, or a dense/binary lookup in a
+// every other participating list (binary search in a child range, or a table/binary lookup in a
 // level-0 list, which also opens that relation's child range). Failed candidates simply get no
 // children, so no compaction is needed between levels. Any query over two-attribute relations
 // with up to 4 attributes and 6 relations runs here (triangle, 4-clique, multi-way).
+#include <type_traits>
+
 #include "lftj_dfs.h"
 
 namespace amj {
+
+// The wave's LDS stage of the cooperative last level (kernels of a compiled shape only).
+template <uint64_t SH>
+__device__ __forceinline__ lds_i32* coop_stage() {
+  if constexpr (SH != 0) {
+    __shared__ int32_t coop_mem[kWavesPerBlock * kStageInts];
+    return (lds_i32*)&coop_mem[(threadIdx.x / kWave) * kStageInts];
+  } else {
+    return nullptr;
+  }
+}
+
+uint64_t lftj_static_shape(const LPlan& P) {
+  if (!P.off32) return 0;
+  uint64_t c = 0;
+  for (int r = 0; r < P.n_rels; ++r) {
+    c |= 1ull << (12 * P.x[r] + r);
+    c |= 1ull << (12 * P.y[r] + 6 + r);
+  }
+  if (P.n_attrs == 3 && c == kShapeTri) return c;
+  if (P.n_attrs == 4 && c == kShapeFour) return c;
+  return 0;
+}
 
 // Unit weights for the launch order: the squared candidate count at attribute 1, summed over
 // the unit's roots (what a root's second level costs when every candidate opens a list of its
 // own), as the key (~bits(float(weight)) << 32) | unit (the bits of a non-negative float order
 // like its value), so an ascending sort puts the heaviest units first and the dispatcher starts
 // them before the light ones.
-template <int A>
+template <int A, uint64_t SH>
 __global__ __launch_bounds__(kBlock) void lftj_unit_weight_kernel(const LPlan P,
                                                                   uint64_t* __restrict__ keys,
                                                                   uint64_t units) {
   const uint64_t k = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
   if (k >= units) return;  // wave-uniform; no barriers in this kernel
   LLv<A> l0;
-  level0(P, k, l0);
+  level0<A, SH>(P, k, l0);
   const uint64_t w = wave_sum_u64((uint64_t)l0.dlen * (uint64_t)l0.dlen);
   if (lane_id() == 0) keys[k] = weight_key(w, k);
 }
 
 // Count pass: counts[unit] for every unit, taken in `order` (nullptr: number order).
-template <int A>
+template <int A, uint64_t SH>
 __global__ __launch_bounds__(kBlock) void lftj_count_kernel(const LPlan P,
                                                             uint64_t* __restrict__ counts,
                                                             uint64_t units,
@@ -47,9 +72,9 @@ __global__ __launch_bounds__(kBlock) void lftj_count_kernel(const LPlan P,
   if (k >= units) return;  // no barriers in this kernel
   const uint64_t unit = unit_of(order, k);
   LLv<A> l0;
-  level0(P, unit, l0);
+  level0<A, SH>(P, unit, l0);
   CountSink sink;
-  if (l0.T) descend<A, 1>(P, l0, sink);
+  if (l0.T) descend<A, 1, SH>(P, l0, sink, coop_stage<SH>());
   if (lane_id() == 0) counts[unit] = sink.count;
 }
 
@@ -57,7 +82,7 @@ __global__ __launch_bounds__(kBlock) void lftj_count_kernel(const LPlan P,
 // offset, i < per_wave. A few roots per wave instead of a unit of 64: the launch lasts as long
 // as its slowest wave, and on a skewed graph a unit of 64 roots costs several times its
 // heaviest root.
-template <int A>
+template <int A, uint64_t SH>
 __global__ __launch_bounds__(kBlock) void lftj_sample_kernel(const LPlan P,
                                                              uint64_t* __restrict__ counts,
                                                              uint64_t waves, uint32_t per_wave,
@@ -67,13 +92,13 @@ __global__ __launch_bounds__(kBlock) void lftj_sample_kernel(const LPlan P,
   const uint32_t lane = (uint32_t)lane_id();
   const uint64_t r = (k * per_wave + lane) * stride + offset;
   LLv<A> l0;
-  level0_at(P, r, lane < per_wave && r < P.roots, l0);
+  level0_at<A, SH>(P, r, lane < per_wave && r < P.roots, l0);
   CountSink sink;
-  if (l0.T) descend<A, 1>(P, l0, sink);
+  if (l0.T) descend<A, 1, SH>(P, l0, sink, coop_stage<SH>());
   if (lane == 0) counts[k] = sink.count;
 }
 
-template <int A, int MODE>
+template <int A, uint64_t SH, int MODE>
 __global__ __launch_bounds__(kBlock) void lftj_write_kernel(const LPlan P,
                                                             const uint64_t* __restrict__ base,
                                                             const LOut cols,
@@ -87,108 +112,171 @@ __global__ __launch_bounds__(kBlock) void lftj_write_kernel(const LPlan P,
   if (k < units) {
     const uint64_t unit = unit_of(order, k);
     LLv<A> l0;
-    level0(P, unit, l0);
+    level0<A, SH>(P, unit, l0);
     start = base ? uniform(base[unit]) : 0;
     sink.out_base = start;
-    if (l0.T) descend<A, 1>(P, l0, sink);
+    if (l0.T) descend<A, 1, SH>(P, l0, sink, coop_stage<SH>());
   }
   // out_base is uniform: lane 0 carries the wave's count
   block_sum_store2(sink.csum, lane_id() == 0 ? sink.out_base - start : 0,
                    &block_sums[blockIdx.x], block_counts ? &block_counts[blockIdx.x] : nullptr);
 }
 
-template <int A, bool HASH>
+// Direct output, rows reserved `chunk` at a time (see DirectSink); every wave records the unused
+// tail of its last chunk in tails[] and per-block result counts go to block_counts.
+template <int A, uint64_t SH, bool HASH>
 __global__ __launch_bounds__(kBlock) void lftj_direct_kernel(const LPlan P, const LOut cols,
-                                                             uint64_t capacity,
+                                                             uint64_t capacity, uint32_t chunk,
                                                              unsigned long long* __restrict__ rows,
+                                                             uint64_t* __restrict__ tails,
                                                              uint64_t* __restrict__ block_sums,
+                                                             uint64_t* __restrict__ block_counts,
                                                              uint64_t units,
                                                              const uint64_t* __restrict__ order) {
   __shared__ int32_t stage_mem[kWavesPerBlock * A * kLftjDirectRows];
   const uint64_t k = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
-  DirectSink<HASH> sink{cols, capacity, rows,
-                        (lds_i32*)&stage_mem[(threadIdx.x / kWave) * A * kLftjDirectRows], 0u,
-                        A, 0};
+  DirectSink<A, HASH, true> sink{cols, capacity, rows,
+                                 (lds_i32*)&stage_mem[(threadIdx.x / kWave) * A * kLftjDirectRows],
+                                 0u, 0};
+  sink.chunk = chunk;
   if (k < units) {
     LLv<A> l0;
-    level0(P, unit_of(order, k), l0);
-    if (l0.T) descend<A, 1>(P, l0, sink);
+    level0<A, SH>(P, unit_of(order, k), l0);
+    if (l0.T) descend<A, 1, SH>(P, l0, sink, coop_stage<SH>());
     sink.flush();
+    sink.tail_of(tails);
   }
-  block_sum_store(sink.csum, &block_sums[blockIdx.x]);
+  block_sum_store2(sink.csum, lane_id() == 0 ? sink.count : 0, &block_sums[blockIdx.x],
+                   &block_counts[blockIdx.x]);
+}
+
+// Fill-up after a chunked direct pass of n results whose chunks end at `reserved`: the rows at
+// [n, reserved) outside the chunks' unused tails move, in order, to the tail slots below n, so
+// the n results are dense. tail[c] is chunk c's unused tail (its last tail[c] rows) and tpre
+// its exclusive prefix sum, for the reserved / chunk chunks. Reads and writes never overlap
+// (sources lie at or past n, destinations below it).
+__global__ __launch_bounds__(kBlock) void chunk_fill_kernel(const LOut cols, int n_attrs,
+                                                            const uint64_t* __restrict__ tail,
+                                                            const uint64_t* __restrict__ tpre,
+                                                            uint64_t n_chunks, uint32_t chunk,
+                                                            uint64_t n, uint64_t reserved) {
+  // tail slots below n: every tail of the chunks before n's chunk, and the part of that
+  // chunk's tail below n
+  const uint64_t cn = n / chunk;
+  uint64_t below = cn < n_chunks ? tpre[cn] : tpre[n_chunks - 1] + tail[n_chunks - 1];
+  if (cn < n_chunks) {
+    const uint64_t keep = chunk - tail[cn], o = n % chunk;
+    below += o > keep ? o - keep : 0;
+  }
+  for (uint64_t p = n + (uint64_t)blockIdx.x * kBlock + threadIdx.x; p < reserved;
+       p += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t c = p / chunk, o = p % chunk;
+    if (o >= chunk - tail[c]) continue;  // p is in a tail
+    // results in [n, p): positions minus the tail slots among them
+    const uint64_t r = (p - n) - (tpre[c] - below);
+    // the r-th tail slot: chunk c2 = last chunk with tpre[c2] <= r
+    uint64_t lo = 0, hi = cn < n_chunks ? cn + 1 : n_chunks;
+    while (hi - lo > 1) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (tpre[mid] <= r) lo = mid;
+      else hi = mid;
+    }
+    const uint64_t dst = lo * chunk + (chunk - tail[lo]) + (r - tpre[lo]);
+    for (int a = 0; a < n_attrs; ++a) cols.c[a][dst] = cols.c[a][p];
+  }
 }
 
 uint64_t lftj_units(const LPlan& P) { return (P.roots + kWave - 1) / kWave; }
 
-static hipError_t launch_count(const LPlan& P, uint64_t* counts, hipStream_t s, uint64_t units,
-                               const uint64_t* order) {
-  if (units == 0) return hipSuccess;
-  const dim3 grid((unsigned)((units + kWavesPerBlock - 1) / kWavesPerBlock));
-#define LC(AV)                                                                                \
-  hipLaunchKernelGGL(lftj_count_kernel<AV>, grid, dim3(kBlock), 0, s, P, counts, units, order)
+template <int V>
+using IntC = std::integral_constant<int, V>;
+template <uint64_t V>
+using ShapeC = std::integral_constant<uint64_t, V>;
+
+// f(IntC<A>, ShapeC<SH>) for the plan's attribute count and compiled shape (SH = 0: the masks
+// are read at run time).
+template <typename F>
+static hipError_t dispatch(const LPlan& P, F&& f) {
+  const uint64_t sh = lftj_static_shape(P);
   switch (P.n_attrs) {
-    case 2: LC(2); break;
-    case 3: LC(3); break;
-    case 4: LC(4); break;
+    case 2: f(IntC<2>{}, ShapeC<0>{}); break;
+    case 3:
+      if (sh == kShapeTri) f(IntC<3>{}, ShapeC<kShapeTri>{});
+      else f(IntC<3>{}, ShapeC<0>{});
+      break;
+    case 4:
+      if (sh == kShapeFour) f(IntC<4>{}, ShapeC<kShapeFour>{});
+      else f(IntC<4>{}, ShapeC<0>{});
+      break;
     default: return hipErrorInvalidValue;
   }
-#undef LC
   return hipGetLastError();
+}
+
+static dim3 grid_of(uint64_t waves) {
+  return dim3((unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock));
 }
 
 hipError_t launch_lftj_sample(const LPlan& P, uint64_t* counts, uint64_t waves,
                               uint32_t per_wave, uint64_t stride, uint64_t offset,
                               hipStream_t s) {
   if (waves == 0) return hipSuccess;
-  const dim3 grid((unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock));
-#define LS(AV)                                                                                 \
-  hipLaunchKernelGGL(lftj_sample_kernel<AV>, grid, dim3(kBlock), 0, s, P, counts, waves, per_wave, \
-                     stride, offset)
-  switch (P.n_attrs) {
-    case 2: LS(2); break;
-    case 3: LS(3); break;
-    case 4: LS(4); break;
-    default: return hipErrorInvalidValue;
-  }
-#undef LS
-  return hipGetLastError();
+  return dispatch(P, [&](auto a, auto sh) {
+    hipLaunchKernelGGL((lftj_sample_kernel<decltype(a)::value, decltype(sh)::value>),
+                       grid_of(waves), dim3(kBlock), 0, s, P, counts, waves, per_wave, stride,
+                       offset);
+  });
 }
 
 hipError_t launch_lftj_count(const LPlan& P, uint64_t* counts, const uint64_t* order,
                              hipStream_t s) {
-  return launch_count(P, counts, s, lftj_units(P), order);
+  const uint64_t units = lftj_units(P);
+  if (units == 0) return hipSuccess;
+  return dispatch(P, [&](auto a, auto sh) {
+    hipLaunchKernelGGL((lftj_count_kernel<decltype(a)::value, decltype(sh)::value>),
+                       grid_of(units), dim3(kBlock), 0, s, P, counts, units, order);
+  });
 }
 
 hipError_t launch_lftj_unit_weights(const LPlan& P, uint64_t* keys, hipStream_t s) {
   const uint64_t units = lftj_units(P);
   if (units == 0) return hipSuccess;
-  const dim3 grid((unsigned)((units + kWavesPerBlock - 1) / kWavesPerBlock));
-  switch (P.n_attrs) {
-    case 2: hipLaunchKernelGGL(lftj_unit_weight_kernel<2>, grid, dim3(kBlock), 0, s, P, keys, units); break;
-    case 3: hipLaunchKernelGGL(lftj_unit_weight_kernel<3>, grid, dim3(kBlock), 0, s, P, keys, units); break;
-    case 4: hipLaunchKernelGGL(lftj_unit_weight_kernel<4>, grid, dim3(kBlock), 0, s, P, keys, units); break;
-    default: return hipErrorInvalidValue;
-  }
-  return hipGetLastError();
+  return dispatch(P, [&](auto a, auto sh) {
+    hipLaunchKernelGGL((lftj_unit_weight_kernel<decltype(a)::value, decltype(sh)::value>),
+                       grid_of(units), dim3(kBlock), 0, s, P, keys, units);
+  });
 }
 
-hipError_t launch_lftj_direct(const LPlan& P, const LOut& cols, uint64_t capacity, bool hash,
-                              unsigned long long* rows, uint64_t* block_sums,
+hipError_t launch_lftj_direct(const LPlan& P, const LOut& cols, uint64_t capacity,
+                              uint32_t chunk, bool hash, unsigned long long* rows,
+                              uint64_t* tails, uint64_t* block_sums, uint64_t* block_counts,
                               uint64_t* n_blocks, const uint64_t* order, hipStream_t s) {
   const uint64_t units = lftj_units(P);
   const uint64_t blocks = units == 0 ? 1 : (units + kWavesPerBlock - 1) / kWavesPerBlock;
   *n_blocks = blocks;
-  const dim3 grid((unsigned)blocks);
-#define LD(AV, H)                                                                             \
-  hipLaunchKernelGGL((lftj_direct_kernel<AV, H>), grid, dim3(kBlock), 0, s, P, cols, capacity, \
-                     rows, block_sums, units, order)
-  switch (P.n_attrs) {
-    case 2: if (hash) LD(2, true); else LD(2, false); break;
-    case 3: if (hash) LD(3, true); else LD(3, false); break;
-    case 4: if (hash) LD(4, true); else LD(4, false); break;
-    default: return hipErrorInvalidValue;
-  }
-#undef LD
+  if (chunk == 0 || chunk % kLftjDirectRows != 0) return hipErrorInvalidValue;
+  return dispatch(P, [&](auto a, auto sh) {
+    constexpr int AV = decltype(a)::value;
+    constexpr uint64_t SV = decltype(sh)::value;
+    if (hash)
+      hipLaunchKernelGGL((lftj_direct_kernel<AV, SV, true>), dim3((unsigned)blocks), dim3(kBlock),
+                         0, s, P, cols, capacity, chunk, rows, tails, block_sums, block_counts,
+                         units, order);
+    else
+      hipLaunchKernelGGL((lftj_direct_kernel<AV, SV, false>), dim3((unsigned)blocks),
+                         dim3(kBlock), 0, s, P, cols, capacity, chunk, rows, tails, block_sums,
+                         block_counts, units, order);
+  });
+}
+
+hipError_t launch_chunk_fill(const LOut& cols, int n_attrs, const uint64_t* tail,
+                             const uint64_t* tpre, uint64_t n_chunks, uint32_t chunk, uint64_t n,
+                             uint64_t reserved, hipStream_t s) {
+  if (reserved <= n || n_chunks == 0) return hipSuccess;
+  uint64_t g = (reserved - n + kBlock - 1) / kBlock;
+  if (g > 65536) g = 65536;
+  hipLaunchKernelGGL(chunk_fill_kernel, dim3((unsigned)g), dim3(kBlock), 0, s, cols, n_attrs,
+                     tail, tpre, n_chunks, chunk, n, reserved);
   return hipGetLastError();
 }
 
@@ -198,24 +286,21 @@ hipError_t launch_lftj_write(const LPlan& P, const uint64_t* base, const LOut& c
   const uint64_t units = lftj_units(P);
   const uint64_t blocks = units == 0 ? 1 : (units + kWavesPerBlock - 1) / kWavesPerBlock;
   *n_blocks = blocks;
-  const dim3 grid((unsigned)blocks);
   const int mode = cols.c[0] ? (hash ? MODE_WRITE_HASH : MODE_WRITE) : MODE_HASH;
-#define LW(AV, M) \
-  hipLaunchKernelGGL((lftj_write_kernel<AV, M>), grid, dim3(kBlock), 0, s, P, base, cols, block_sums, \
-                     block_counts, units, order)
-#define LWA(AV)                                              \
-  if (mode == MODE_WRITE) LW(AV, MODE_WRITE);                \
-  else if (mode == MODE_WRITE_HASH) LW(AV, MODE_WRITE_HASH); \
-  else LW(AV, MODE_HASH)
-  switch (P.n_attrs) {
-    case 2: LWA(2); break;
-    case 3: LWA(3); break;
-    case 4: LWA(4); break;
-    default: return hipErrorInvalidValue;
-  }
-#undef LWA
-#undef LW
-  return hipGetLastError();
+  return dispatch(P, [&](auto a, auto sh) {
+    constexpr int AV = decltype(a)::value;
+    constexpr uint64_t SV = decltype(sh)::value;
+    const dim3 grid((unsigned)blocks);
+    if (mode == MODE_WRITE)
+      hipLaunchKernelGGL((lftj_write_kernel<AV, SV, MODE_WRITE>), grid, dim3(kBlock), 0, s, P,
+                         base, cols, block_sums, block_counts, units, order);
+    else if (mode == MODE_WRITE_HASH)
+      hipLaunchKernelGGL((lftj_write_kernel<AV, SV, MODE_WRITE_HASH>), grid, dim3(kBlock), 0, s,
+                         P, base, cols, block_sums, block_counts, units, order);
+    else
+      hipLaunchKernelGGL((lftj_write_kernel<AV, SV, MODE_HASH>), grid, dim3(kBlock), 0, s, P,
+                         base, cols, block_sums, block_counts, units, order);
+  });
 }
 
 }  // namespace amj

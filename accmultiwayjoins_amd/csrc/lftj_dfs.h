@@ -9,7 +9,7 @@
 namespace amj {
 
 // Level 0 from root r of driver0 on every lane where `valid`.
-template <int A>
+template <int A, uint64_t SH = 0>
 __device__ __forceinline__ void level0_at(const LPlan& P, uint64_t r, bool valid, LLv<A>& l) {
   sfor<0, A>([&](auto a) { l.b[a] = 0; });
   sfor<0, kR>([&](auto q) { l.lo[q] = 0u; l.hi[q] = 0u; });
@@ -17,17 +17,20 @@ __device__ __forceinline__ void level0_at(const LPlan& P, uint64_t r, bool valid
   const LRel& d = P.rel[P.driver0];
   if (valid) v = d.d0[r];
   bool ok = valid;
-  if (valid) ok = check_and_open(P, 0, v, -1, true, l);
+  if (valid) {
+    if constexpr (SH != 0) ok = check_and_open_fixed<A, SH, 0>(P, v, -1, l);
+    else ok = check_and_open(P, 0, v, -1, true, l);
+  }
   l.b[0] = v;
-  choose_driver(P, 1, ok, l);
+  choose_driver<A, SH>(P, 1, ok, l);
   finish(l);
 }
 
 // Level 0 of a unit: 64 consecutive level-0 values of driver0.
-template <int A>
+template <int A, uint64_t SH = 0>
 __device__ __forceinline__ void level0(const LPlan& P, uint64_t unit, LLv<A>& l) {
   const uint64_t r = unit * kWave + (uint64_t)lane_id();
-  level0_at(P, r, r < P.roots, l);
+  level0_at<A, SH>(P, r, r < P.roots, l);
 }
 
 // What happens to the candidates that pass the last attribute (`ok` per lane, bindings in b):
@@ -79,51 +82,88 @@ struct WriteSink {
   }
 };
 
-// Direct output: results gather in a per-wave LDS buffer of kLftjDirectRows rows and leave
-// with one atomic reservation per full buffer; rows at or past `capacity` are not written (the
-// reserved total then tells the host to run again with an exact allocation).
+// Direct output: results gather in a per-wave LDS buffer of kLftjDirectRows rows; rows at or
+// past `capacity` are not written (the reserved total then tells the host to run again with a
+// larger allocation).
+//   * CHUNKED = false: one atomic reservation on the launch's row counter per full buffer.
+//   * CHUNKED = true: rows are reserved `chunk` at a time (a multiple of kLftjDirectRows) and a
+//     wave fills its chunk buffer by buffer, so the shared counter sees one update per `chunk`
+//     rows instead of one per buffer (one counter updated by every wave serialises them: about
+//     14 ns per update on this device, which bounded a whole triangle pass). Only a wave's last
+//     chunk is left partly unused; the wave records that tail (`tail_of`) and a fill-up pass
+//     moves the rows found past the result count into the tails below it (chunk_fill_kernel).
 constexpr uint32_t kLftjDirectRows = 256;
 typedef __attribute__((address_space(3))) int32_t lds_i32;
 
-template <bool HASH>
+// The checksum is taken from the buffer when it is flushed, with every lane busy, rather than
+// per result on the lanes whose candidate passed.
+template <int A, bool HASH, bool CHUNKED = false>
 struct DirectSink {
   LOut cols;
   uint64_t capacity;
-  unsigned long long* rows;  // reserved rows of the whole launch
+  unsigned long long* rows;  // rows reserved by the whole launch
   lds_i32* buf;              // attribute a of row i at buf[a * kLftjDirectRows + i]
   uint32_t fill;             // rows held (uniform)
-  int n_attrs;
   uint64_t csum;
+  uint32_t chunk = 0;        // CHUNKED: rows per reservation
+  uint64_t cur = ~0ull;      // CHUNKED: first row of the wave's current chunk (~0: none yet)
+  uint32_t used = 0;         // CHUNKED: rows of the current chunk taken
+  uint64_t count = 0;        // rows passed to the sink (uniform)
 
   __device__ __forceinline__ void flush() {
     if (fill == 0) return;
-    unsigned long long r0 = 0;
-    if (lane_id() == 0)
-      r0 = __hip_atomic_fetch_add(rows, (unsigned long long)fill, __ATOMIC_RELAXED,
-                                  __HIP_MEMORY_SCOPE_AGENT);
-    const uint64_t base = uniform((uint64_t)r0);
+    // rows [0, na) of the buffer go to base_a + i, rows [na, fill) to base_b + i - na
+    uint64_t base_a = 0, base_b = 0;
+    uint32_t na = fill;
+    if constexpr (CHUNKED) {
+      const uint32_t rem = cur == ~0ull ? 0u : chunk - used;
+      na = fill < rem ? fill : rem;
+      base_a = cur + used;
+      if (fill > rem) {
+        unsigned long long r0 = 0;
+        if (lane_id() == 0)
+          r0 = __hip_atomic_fetch_add(rows, (unsigned long long)chunk, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+        base_b = uniform((uint64_t)r0);
+        cur = base_b;
+        used = fill - na;
+      } else {
+        used += fill;
+      }
+    } else {
+      unsigned long long r0 = 0;
+      if (lane_id() == 0)
+        r0 = __hip_atomic_fetch_add(rows, (unsigned long long)fill, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
+      base_a = uniform((uint64_t)r0);
+    }
+    count += fill;
     __asm__ __volatile__("" ::: "memory");  // LDS ops of one wave complete in order
     for (uint32_t i = (uint32_t)lane_id(); i < fill; i += kWave) {
-      const uint64_t pos = base + i;
-      if (pos < capacity)
-        sfor<0, kMaxLftjAttrs>([&](auto a) {
-          if ((int)a < n_attrs) cols.c[a][pos] = buf[a * kLftjDirectRows + i];
-        });
+      const uint64_t pos = i < na ? base_a + i : base_b + (i - na);
+      int32_t w[kMaxAttrs];
+      sfor<0, kMaxAttrs>([&](auto a) {
+        if constexpr ((int)a < A) w[a] = buf[(int)a * kLftjDirectRows + i];
+        else w[a] = 0;
+      });
+      if (HASH) csum += hash_tuple(w, A);
+      if (pos < capacity) sfor<0, A>([&](auto a) { cols.c[a][pos] = w[a]; });
     }
     __asm__ __volatile__("" ::: "memory");
     fill = 0;
   }
 
-  template <int A>
+  // CHUNKED, after the last flush: the unused tail of the wave's last chunk, recorded by lane 0
+  // as tail[chunk index] when the chunk lies inside the allocation.
+  __device__ __forceinline__ void tail_of(uint64_t* tail) const {
+    if (CHUNKED && cur != ~0ull && lane_id() == 0 && cur + chunk <= capacity)
+      tail[cur / chunk] = chunk - used;
+  }
+
   __device__ __forceinline__ void operator()(bool ok, const int32_t (&b)[A]) {
     const uint64_t m = __ballot(ok);
     const uint32_t n = (uint32_t)__popcll(m);
     if (n == 0) return;
-    if (HASH && ok) {
-      int32_t w[kMaxAttrs];
-      sfor<0, kMaxAttrs>([&](auto a) { w[a] = (a < A) ? b[(a < A) ? (int)a : 0] : 0; });
-      csum += hash_tuple(w, A);
-    }
     if (fill + n > kLftjDirectRows) flush();
     __asm__ __volatile__("" ::: "memory");
     if (ok) {
@@ -138,7 +178,7 @@ struct DirectSink {
 // Candidates [c, c + 64) of the flattened level `par` (those below `end` are valid) checked
 // for attribute i: ch gets the parent's bindings and child ranges, b[i] = the candidate, and
 // the ranges of the relations rooted at i opened. Returns whether the candidate passed.
-template <int A, int i>
+template <int A, int i, uint64_t SH = 0>
 __device__ __forceinline__ bool lchild(const LPlan& P, const LLv<A>& par, uint32_t c,
                                        uint32_t end, LLv<A>& ch) {
   const uint32_t g = c + (uint32_t)lane_id();
@@ -152,24 +192,184 @@ __device__ __forceinline__ bool lchild(const LPlan& P, const LLv<A>& par, uint32
     ch.lo[q] = __shfl(par.lo[q], p, kWave);
     ch.hi[q] = __shfl(par.hi[q], p, kWave);
   });
-  const int32_t v = valid ? src[k] : 0;
-  const bool drel_root = P.child_mask[i] == 0u;
-  const bool ok = valid && check_and_open(P, i, v, drel, drel_root, ch);
+  const int32_t v = valid ? ld_at<false>(src, k) : 0;
+  bool ok = valid;
+  if constexpr (SH != 0) {
+    ok = ok && check_and_open_fixed<A, SH, i>(P, v, drel, ch);
+  } else {
+    const bool drel_root = P.child_mask[i] == 0u;
+    ok = ok && check_and_open(P, i, v, drel, drel_root, ch);
+  }
   ch.b[i] = v;
   return ok;
 }
 
-template <int A, int i, typename Sink>
-__device__ __forceinline__ void descend(const LPlan& P, const LLv<A>& par, Sink& sink) {
-  for (uint32_t c = 0; c < par.T; c += (uint32_t)kWave) {
-    LLv<A> ch;
-    const bool ok = lchild<A, i>(P, par, c, par.T, ch);
-    if constexpr (i == A - 1) {
+// Last attribute of a compiled shape. An entry whose driver list holds at least
+// kCoopMinCands candidates and whose other child lists (every child list of the attribute but
+// the driver's) hold at most kStageInts values together is run by the whole wave: the other
+// lists are copied to the wave's LDS stage once, and each chunk of 64 candidates is checked
+// there by binary searches of uniform length (one global load per candidate instead of a
+// chain of dependent ones). The other entries go through the flattened enumeration.
+constexpr uint32_t kCoopMinCands = 32;
+constexpr uint32_t kStageInts = 1024;  // per wave
+
+__device__ __forceinline__ uint32_t readlane_u32(uint32_t v, int lane) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
+}
+__device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int lane) {
+  return (uint64_t)readlane_u32((uint32_t)v, lane) |
+         ((uint64_t)readlane_u32((uint32_t)(v >> 32), lane) << 32);
+}
+
+// Are v[j] in the ascending run st[0, n) of LDS (n the same on every lane)? The K searches
+// advance together, so K reads are in flight at every step; ok[j] is cleared for a miss.
+template <int K>
+__device__ __forceinline__ void lds_has(const lds_i32* st, uint32_t n, const int32_t (&v)[K],
+                                        bool (&ok)[K]) {
+  uint32_t b[K];
+  sfor<0, K>([&](auto j) { b[j] = 0u; });
+  if (n == 0) {
+    sfor<0, K>([&](auto j) { ok[j] = false; });
+    return;
+  }
+  while (n > 1) {
+    const uint32_t half = n >> 1;
+    int32_t x[K];
+    sfor<0, K>([&](auto j) { x[j] = st[b[j] + half]; });
+    sfor<0, K>([&](auto j) { b[j] = x[j] <= v[j] ? b[j] + half : b[j]; });
+    n -= half;
+  }
+  sfor<0, K>([&](auto j) { ok[j] = ok[j] && st[b[j]] == v[j]; });
+}
+
+// Chunks of 64 candidates a wave checks at once in the cooperative last level.
+constexpr int kCoopChunks = 4;
+
+template <int A, int I, uint64_t SH, typename Sink>
+__device__ __forceinline__ void descend_last_fixed(const LPlan& P, const LLv<A>& par, Sink& sink,
+                                                   lds_i32* stage) {
+  constexpr uint32_t cm = (uint32_t)(SH >> (12 * I + 6)) & 63u;
+  constexpr int nc = bit_count(cm);
+  static_assert(nc >= 2, "the last attribute has a driver and at least one list to check");
+  constexpr int no = nc - 1;
+  constexpr int K = kCoopChunks;
+  // this lane's other lists, as in check_and_open_fixed
+  int di = 0;
+  sfor<1, nc>([&](auto j) {
+    if (par.drel == nth_bit(cm, (int)j)) di = (int)j;
+  });
+  const int32_t* optr[no];
+  uint32_t olen[no];
+  uint32_t total = 0;
+  sfor<0, no>([&](auto k) {
+    constexpr int ra = nth_bit(cm, (int)k), rb = nth_bit(cm, (int)k + 1);
+    const bool next = (int)k >= di;
+    const uint32_t lo = next ? par.lo[rb] : par.lo[ra];
+    const uint32_t hi = next ? par.hi[rb] : par.hi[ra];
+    optr[k] = (next ? P.rel[rb].d1 : P.rel[ra].d1) + lo;
+    olen[k] = hi - lo;
+    total += hi - lo;
+  });
+  const bool coop = par.dlen >= kCoopMinCands && total <= kStageInts;
+
+  // entries with few candidates (or long other lists): flattened
+  {
+    LLv<A> sm = par;
+    sm.dlen = coop ? 0u : par.dlen;
+    finish(sm);
+    for (uint32_t c = 0; c < sm.T; c += (uint32_t)kWave) {
+      LLv<A> ch;
+      const bool ok = lchild<A, I, SH>(P, sm, c, sm.T, ch);
       sink(ok, ch.b);
-    } else {
-      choose_driver(P, i + 1, ok, ch);
-      finish(ch);
-      if (ch.T) descend<A, i + 1>(P, ch, sink);
+    }
+  }
+
+  // the others, one entry at a time; consecutive entries often check the same list (children
+  // of one parent value), which then stays staged
+  uint64_t m = __ballot(coop);
+  const uint32_t lane = (uint32_t)lane_id();
+  uint64_t staged[no];
+  uint32_t soff[no], slen[no];
+  sfor<0, no>([&](auto k) {
+    staged[k] = 0ull;
+    soff[k] = 0u;
+    slen[k] = 0u;
+  });
+  while (m) {
+    const int p = __ffsll((unsigned long long)m) - 1;
+    m &= m - 1;
+    const int32_t* dp = (const int32_t*)readlane_u64((uint64_t)par.dptr, p);
+    const uint32_t dl = readlane_u32(par.dlen, p);
+    const int32_t* src[no];
+    uint32_t n[no];
+    bool same = true;
+    sfor<0, no>([&](auto k) {
+      src[k] = (const int32_t*)readlane_u64((uint64_t)optr[k], p);
+      n[k] = readlane_u32(olen[k], p);
+      same = same && (uint64_t)src[k] == staged[k] && n[k] == slen[k];
+    });
+    if (!same) {
+      __asm__ __volatile__("" ::: "memory");  // earlier reads of the stage are done (in order)
+      uint32_t at = 0;
+      sfor<0, no>([&](auto k) {
+        for (uint32_t j0 = 0; j0 < n[k]; j0 += 4u * kWave) {
+          int32_t t[4];
+          sfor<0, 4>([&](auto u) {
+            const uint32_t j = j0 + (uint32_t)u * kWave + lane;
+            t[u] = j < n[k] ? ld_at<false>(src[k], j) : 0;
+          });
+          sfor<0, 4>([&](auto u) {
+            const uint32_t j = j0 + (uint32_t)u * kWave + lane;
+            if (j < n[k]) stage[at + j] = t[u];
+          });
+        }
+        staged[k] = (uint64_t)src[k];
+        soff[k] = at;
+        slen[k] = n[k];
+        at += n[k];
+      });
+      __asm__ __volatile__("" ::: "memory");  // LDS ops of one wave complete in order
+    }
+    int32_t b[A];
+    sfor<0, A>([&](auto a) {
+      if constexpr ((int)a < I) b[a] = (int32_t)readlane_u32((uint32_t)par.b[a], p);
+      else b[a] = 0;
+    });
+    for (uint32_t c = 0; c < dl; c += (uint32_t)(K * kWave)) {
+      int32_t v[K];
+      bool ok[K];
+      sfor<0, K>([&](auto j) {
+        const uint32_t g = c + (uint32_t)j * kWave + lane;
+        ok[j] = g < dl;
+        v[j] = ok[j] ? ld_at<false>(dp, g) : 0;
+      });
+      sfor<0, no>([&](auto k) { lds_has<K>(stage + soff[k], slen[k], v, ok); });
+      sfor<0, K>([&](auto j) {
+        if (c + (uint32_t)j * kWave < dl) {
+          b[I] = v[j];
+          sink(ok[j], b);
+        }
+      });
+    }
+  }
+}
+
+template <int A, int i, uint64_t SH = 0, typename Sink>
+__device__ __forceinline__ void descend(const LPlan& P, const LLv<A>& par, Sink& sink,
+                                        lds_i32* stage = nullptr) {
+  if constexpr (SH != 0 && i == A - 1) {
+    descend_last_fixed<A, i, SH>(P, par, sink, stage);
+  } else {
+    for (uint32_t c = 0; c < par.T; c += (uint32_t)kWave) {
+      LLv<A> ch;
+      const bool ok = lchild<A, i, SH>(P, par, c, par.T, ch);
+      if constexpr (i == A - 1) {
+        sink(ok, ch.b);
+      } else {
+        choose_driver<A, SH>(P, i + 1, ok, ch);
+        finish(ch);
+        if (ch.T) descend<A, i + 1, SH>(P, ch, sink, stage);
+      }
     }
   }
 }

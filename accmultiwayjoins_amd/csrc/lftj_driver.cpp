@@ -45,6 +45,7 @@ static int make_lplan(amj_ctx_t* ctx, const std::vector<LUse>& uses, const uint3
       }
   P->n_attrs = (int32_t)na;
   P->n_rels = (int32_t)uses.size();
+  P->off32 = 1;
   uint64_t widest = 0;  // longest candidate list any frontier entry can take
   for (size_t r = 0; r < uses.size(); ++r) {
     const int px = pos_of(uses[r].x_id), py = pos_of(uses[r].y_id);
@@ -63,6 +64,9 @@ static int make_lplan(amj_ctx_t* ctx, const std::vector<LUse>& uses, const uint3
     P->root_mask[px] |= 1u << r;
     P->child_mask[py] |= 1u << r;
     widest = std::max<uint64_t>(widest, uses[r].t->dev.max_run);
+    const TrieDev& d = uses[r].t->dev;
+    if ((uint64_t)d.n0 + 1 >= (1ull << 30) || d.n1 >= (1ull << 30) || d.vrange_n >= (1u << 29))
+      P->off32 = 0;
   }
   if (P->root_mask[0] == 0u) {
     set_error(ctx, AMJ_ERR_ARG, "no relation starts at the first attribute of attr_order");
@@ -151,27 +155,45 @@ static int sample_capacity(amj_ctx_t* ctx, const LPlan& P, uint64_t* capacity) {
   return AMJ_OK;
 }
 
+// Rows per reservation of the direct pass: the largest power of two in [256, 4096] whose
+// worst-case unused tails (one per wave) stay within 1/16 of the expected output.
+static uint32_t direct_chunk(uint64_t expected, uint64_t waves) {
+  uint32_t chunk = 256;
+  while (chunk < 4096 && waves * (uint64_t)chunk * 2 * 16 <= expected) chunk *= 2;
+  return chunk;
+}
+
 static int run_lftj_direct(amj_ctx_t* ctx, const LPlan& P, const amj_opts_t* o,
                            amj_result_t* res) {
   hipStream_t s = ctx->stream;
   const uint64_t units = lftj_units(P);
   AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[2], s));
-  uint64_t capacity = o->dro_capacity_rows;
+  uint64_t expected = o->dro_capacity_rows;
   uint32_t passes = 0;
-  if (capacity == 0 && units > 0) {
-    AMJ_TRY(sample_capacity(ctx, P, &capacity));
+  bool slack = false;  // a sampled size gets room for the unused chunk tails
+  if (expected == 0 && units > 0) {
+    AMJ_TRY(sample_capacity(ctx, P, &expected));
     ++passes;
+    slack = true;
   }
   const uint64_t* order = nullptr;
   AMJ_TRY(unit_order(ctx, P, &order));
   const uint64_t nb = units == 0 ? 1 : (units + kWavesPerBlock - 1) / kWavesPerBlock;
-  void *bsum = nullptr, *ctl = nullptr;
+  void *bsum = nullptr, *bcnt = nullptr, *ctl = nullptr;
   AMJ_TRY(scratch(ctx, SLOT_BLOCK_SUMS, nb * 8, &bsum));
+  AMJ_TRY(scratch(ctx, SLOT_MISC1, nb * 8, &bcnt));
   AMJ_TRY(scratch(ctx, SLOT_FLAG, 256, &ctl));
   const bool hash = o->checksum != 0;
-  uint64_t rows = 0, nb_used = 0;
+  uint64_t reserved = 0, n = 0, nb_used = 0, capacity = 0;
+  uint32_t chunk = 256;
   double kms = 0.0;
   for (int attempt = 0; attempt < 2; ++attempt) {
+    chunk = direct_chunk(expected, units);
+    const uint64_t want = expected + (slack ? units * (uint64_t)chunk : 0);
+    capacity = (want + chunk - 1) / chunk * chunk;
+    const uint64_t n_chunks = capacity / chunk;
+    void *tail = nullptr, *tpre = nullptr, *stmp = nullptr;
+    AMJ_TRY(scratch(ctx, SLOT_MISC2, std::max<uint64_t>(n_chunks, 1) * 8, &tail));
     LOut out{};
     if (capacity > 0) {
       if (capacity * (uint64_t)P.n_attrs * 4 > budget_left(ctx, o)) {
@@ -182,29 +204,46 @@ static int run_lftj_direct(amj_ctx_t* ctx, const LPlan& P, const amj_opts_t* o,
       }
       AMJ_TRY(alloc_output(ctx, capacity, P.n_attrs, res));
       for (int a = 0; a < P.n_attrs; ++a) out.c[a] = res->cols[a];
+      AMJ_HIP_TRY(ctx, hipMemsetAsync(tail, 0, n_chunks * 8, s));
     }
     AMJ_HIP_TRY(ctx, hipMemsetAsync(ctl, 0, 8, s));
     AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
-    AMJ_HIP_TRY(ctx, launch_lftj_direct(P, out, capacity, hash, (unsigned long long*)ctl,
-                                        (uint64_t*)bsum, &nb_used, order, s));
+    AMJ_HIP_TRY(ctx, launch_lftj_direct(P, out, capacity, chunk, hash, (unsigned long long*)ctl,
+                                        (uint64_t*)tail, (uint64_t*)bsum, (uint64_t*)bcnt,
+                                        &nb_used, order, s));
     AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
-    AMJ_HIP_TRY(ctx, hipMemcpyAsync(&rows, ctl, 8, hipMemcpyDeviceToHost, s));
-    AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
+    AMJ_HIP_TRY(ctx, hipMemcpyAsync(&reserved, ctl, 8, hipMemcpyDeviceToHost, s));
+    AMJ_TRY(finish_checksum(ctx, (const uint64_t*)bcnt, nb_used, &n));
     kms += elapsed_ms(ctx->ev[3], ctx->ev[4]);
     ++passes;
-    if (rows <= capacity) break;
-    amj_result_free(ctx, res);  // overran: the pass counted every result, run again exactly
-    capacity = rows;
+    if (reserved <= capacity) {
+      // the results are the first n rows once the rows past n fill the tails below it
+      const uint64_t used_chunks = reserved / chunk;
+      if (reserved > n) {
+        AMJ_TRY(scratch(ctx, SLOT_MISC3, (used_chunks + 1) * 8, &tpre));
+        AMJ_TRY(scratch(ctx, SLOT_SCAN_TEMP, scan_temp_elems(used_chunks) * 8, &stmp));
+        AMJ_HIP_TRY(ctx, exclusive_scan_u64((const uint64_t*)tail, (uint64_t*)tpre, used_chunks,
+                                            (uint64_t*)stmp, s));
+        AMJ_HIP_TRY(ctx, launch_chunk_fill(out, P.n_attrs, (const uint64_t*)tail,
+                                           (const uint64_t*)tpre, used_chunks, chunk, n,
+                                           reserved, s));
+      }
+      break;
+    }
+    // overran: the pass counted every result; run again with room for them and the tails
+    amj_result_free(ctx, res);
+    expected = n;
+    slack = true;
   }
-  if (rows > capacity) {
-    set_error(ctx, AMJ_ERR_STATE, "direct output overran an exact allocation");
+  if (reserved > capacity) {
+    set_error(ctx, AMJ_ERR_STATE, "direct output overran an allocation sized for it");
     return AMJ_ERR_STATE;
   }
   if (hash) AMJ_TRY(finish_checksum(ctx, (const uint64_t*)bsum, nb_used, &res->checksum));
   AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[5], s));
   AMJ_HIP_TRY(ctx, hipEventSynchronize(ctx->ev[5]));
-  res->count = rows;
-  res->rows_written = rows;
+  res->count = n;
+  res->rows_written = n;
   res->probe_ms = elapsed_ms(ctx->ev[2], ctx->ev[5]);
   res->probe_kernel_ms = kms;
   res->probe_kernel_launches = passes;
@@ -414,12 +453,25 @@ static int run_lftj_dfs(amj_ctx_t* ctx, const LPlan& P, const amj_opts_t* o, amj
   return AMJ_OK;
 }
 
-int lftj_execute(amj_ctx_t* ctx, const std::vector<LUse>& uses, const uint32_t* order,
+int lftj_execute(amj_ctx_t* ctx, const std::vector<LUse>& uses_in, const uint32_t* order,
                  uint32_t na, int algo, const amj_opts_t* opts, amj_result_t* res) {
   amj_opts_t def;
   amj_default_opts(&def);
   const amj_opts_t* o = opts ? opts : &def;
   std::memset(res, 0, sizeof(*res));
+  // relations ordered by the positions of their attributes in attr_order (the result set does
+  // not depend on it), so that a triangle or a 4-clique is recognised whatever the trie order
+  std::vector<LUse> uses = uses_in;
+  auto pos_key = [&](const LUse& u) {
+    uint32_t px = na, py = na;
+    for (uint32_t i = 0; i < na; ++i) {
+      if (order[i] == u.x_id) px = i;
+      if (order[i] == u.y_id) py = i;
+    }
+    return px * 64u + py;
+  };
+  std::stable_sort(uses.begin(), uses.end(),
+                   [&](const LUse& a, const LUse& b) { return pos_key(a) < pos_key(b); });
   LPlan P;
   AMJ_TRY(make_lplan(ctx, uses, order, na, &P));
   res->num_cols = na;

@@ -135,7 +135,7 @@ __global__ __launch_bounds__(kBlock) void lftj_ws_kernel(const LPlan P, const LO
   constexpr bool MAT = mode_mat<MODE>();
   constexpr bool HASH = mode_hash<MODE>();
   __shared__ int32_t stage_mem[MAT ? kWavesPerBlock * A * kLftjDirectRows : 1];
-  using Sink = std::conditional_t<MAT, DirectSink<HASH>, CountHashSink<HASH>>;
+  using Sink = std::conditional_t<MAT, DirectSink<A, HASH>, CountHashSink<HASH>>;
   Sink sink{};
   if constexpr (MAT) {
     sink.cols = cols;
@@ -143,7 +143,6 @@ __global__ __launch_bounds__(kBlock) void lftj_ws_kernel(const LPlan P, const LO
     sink.rows = &W.ctl->out_rows;
     sink.buf = (lds_i32*)&stage_mem[(threadIdx.x / kWave) * A * kLftjDirectRows];
     sink.fill = 0u;
-    sink.n_attrs = A;
     sink.csum = 0;
   }
   WsWave q{false, 0u};

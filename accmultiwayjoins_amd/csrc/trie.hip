@@ -8,8 +8,9 @@
 // Here: one stable LSD radix sort of the 64-bit composite ((col0 ^ 2^31) << 32 | (col1 ^ 2^31))
 // orders the pairs by (col0, col1) in signed order; duplicates are dropped (the triejoin
 // intersects sets); a scan of the level-0 head flags gives the distinct values and their
-// offsets. When the level-0 values are small non-negative integers (vertex ids) a dense index
-// value -> row is added so that "is v a level-0 value, and where" is one load.
+// offsets. When the level-0 values are small non-negative integers (vertex ids) a table
+// value -> child range is added so that "is v a level-0 value, and what are its children" is one
+// 8-byte load.
 #include "amj_internal.h"
 #include "lftj.h"
 #include "runtime.h"
@@ -70,12 +71,15 @@ __global__ __launch_bounds__(kBlock) void trie_level0_kernel(const uint64_t* __r
   }
 }
 
-__global__ __launch_bounds__(kBlock) void trie_dense_kernel(const int32_t* __restrict__ d0,
-                                                            uint32_t n0,
-                                                            uint32_t* __restrict__ dense) {
+// vrange[v] = child range of level-0 value v (the table is zeroed first: absent values keep the
+// empty range [0, 0)); every level-0 value has at least one child.
+__global__ __launch_bounds__(kBlock) void trie_vrange_kernel(const int32_t* __restrict__ d0,
+                                                             const uint32_t* __restrict__ off,
+                                                             uint32_t n0,
+                                                             uint2* __restrict__ vrange) {
   for (uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x; j < n0;
        j += (uint64_t)gridDim.x * kBlock)
-    dense[(uint32_t)d0[j]] = (uint32_t)j;
+    vrange[(uint32_t)d0[j]] = make_uint2(off[j], off[j + 1]);
 }
 
 // longest child run (the largest candidate list the triejoin frontier can take from this trie)
@@ -102,7 +106,7 @@ static unsigned grid_for(uint64_t n) {
   return g ? (unsigned)g : 1u;
 }
 
-// Longest child run, optional dense level-0 index, timing and the device view. Expects d0,
+// Longest child run, optional value -> child range table, timing and the device view. Expects d0,
 // off, d1, n0 and n1 set; ev[0] recorded at the start of the build.
 static int finish_trie(amj_ctx_t* ctx, TrieDev* out) {
   hipStream_t s = ctx->stream;
@@ -115,25 +119,25 @@ static int finish_trie(amj_ctx_t* ctx, TrieDev* out) {
     hipLaunchKernelGGL(trie_max_run_kernel, dim3(std::min(grid_for(n0), 1024u)), dim3(kBlock), 0,
                        s, (const uint32_t*)out->off, n0, (uint32_t*)mr);
     AMJ_HIP_TRY(ctx, hipMemcpyAsync(&max_run, mr, 4, hipMemcpyDeviceToHost, s));
-    // dense value -> row index when level-0 values are small non-negative integers
+    // value -> child range table when level-0 values are small non-negative integers
     int32_t lo_v = 0, hi_v = 0;
     AMJ_HIP_TRY(ctx, hipMemcpyAsync(&lo_v, out->d0, 4, hipMemcpyDeviceToHost, s));
     AMJ_HIP_TRY(ctx, hipMemcpyAsync(&hi_v, out->d0 + n0 - 1, 4, hipMemcpyDeviceToHost, s));
     AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
     const uint64_t span = (uint64_t)(uint32_t)hi_v + 1;
     if (lo_v >= 0 && span <= std::max<uint64_t>(4ull * n0, 1ull << 20)) {
-      AMJ_TRY(dmalloc(ctx, (void**)&out->dense, span * 4));
-      out->dense_n = (uint32_t)span;
-      AMJ_HIP_TRY(ctx, hipMemsetAsync(out->dense, 0xff, span * 4, s));
-      hipLaunchKernelGGL(trie_dense_kernel, dim3(grid_for(n0)), dim3(kBlock), 0, s,
-                         (const int32_t*)out->d0, n0, out->dense);
+      AMJ_TRY(dmalloc(ctx, (void**)&out->vrange, span * 8));
+      out->vrange_n = (uint32_t)span;
+      AMJ_HIP_TRY(ctx, hipMemsetAsync(out->vrange, 0, span * 8, s));
+      hipLaunchKernelGGL(trie_vrange_kernel, dim3(grid_for(n0)), dim3(kBlock), 0, s,
+                         (const int32_t*)out->d0, (const uint32_t*)out->off, n0, out->vrange);
     }
   }
   AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[1], s));
   AMJ_HIP_TRY(ctx, hipEventSynchronize(ctx->ev[1]));
   out->build_ms = elapsed_ms(ctx->ev[0], ctx->ev[1]);
   out->max_run = max_run;
-  out->view = LRel{out->d0, out->off, out->d1, out->dense, n0, out->dense_n};
+  out->view = LRel{out->d0, out->off, out->d1, out->vrange, n0, out->vrange_n};
   return AMJ_OK;
 }
 
@@ -239,7 +243,7 @@ void free_trie(amj_ctx_t* ctx, TrieDev* t) {
   dfree(ctx, t->d0, (uint64_t)t->n0 * 4);
   dfree(ctx, t->off, ((uint64_t)t->n0 + 1) * 4);
   dfree(ctx, t->d1, (uint64_t)t->n1 * 4);
-  dfree(ctx, t->dense, (uint64_t)t->dense_n * 4);
+  dfree(ctx, t->vrange, (uint64_t)t->vrange_n * 8);
   *t = TrieDev{};
 }
 
