@@ -237,7 +237,29 @@ hipError_t launch_key_range(const int32_t* key, uint64_t n, uint32_t log_b, uint
 struct ProbeOut {
   int32_t* cols[kMaxAttrs];  // output columns (nullptr: do not materialise)
   uint64_t capacity;         // rows available in every output column
+  // Direct output reserved `chunk` rows at a time (0: one reservation per staging buffer);
+  // tails[c] receives the unused tail of chunk c when it is a wave's last one.
+  uint32_t chunk;
+  uint64_t* tails;
 };
+
+// Rows per reservation of a chunked direct-output pass by `waves` waves expecting `expected`
+// results: 0 (one reservation per buffer) when the output is small, else the largest power of
+// two in [256, 4096] whose worst-case unused tails (one per wave) stay within 1/16 of it.
+inline uint32_t direct_chunk_rows(uint64_t expected, uint64_t waves) {
+  if (expected < waves * 256ull * 16) return 0;
+  uint32_t chunk = 256;
+  while (chunk < 4096 && waves * (uint64_t)chunk * 2 * 16 <= expected) chunk *= 2;
+  return chunk;
+}
+
+// Fill-up after a chunked direct-output pass of n results whose chunks end at `reserved`: the
+// rows at [n, reserved) outside the chunks' unused tails move, in order, to the tail slots below
+// n, so that the first n rows are the results. tail[c] is chunk c's unused tail and tpre its
+// exclusive prefix sum, over the reserved / chunk chunks.
+hipError_t launch_chunk_fill(const ProbeOut& out, int n_attrs, const uint64_t* tail,
+                             const uint64_t* tpre, uint64_t n_chunks, uint32_t chunk, uint64_t n,
+                             uint64_t reserved, hipStream_t s);
 
 // Count pass: per-unit result counts (unit = 64 consecutive probe rows).
 hipError_t launch_probe_count(const Plan& plan, uint64_t* unit_counts, hipStream_t s);

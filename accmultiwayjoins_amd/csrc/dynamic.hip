@@ -150,7 +150,10 @@ __global__ __launch_bounds__(kBlock) void dyn_kernel(const Plan P, const ProbeOu
           if ((int)tt == t) dyn_descend<L, tt, MODE>(P, l, c0, c1, out, A, acc, W);
         });
       });
-  if (MAT) direct_flush<na_of(L)>(out, P.n_attrs, W.st, &A.ctl->out_rows);
+  if (MAT) {
+    direct_flush<na_of(L)>(out, P.n_attrs, W.st, &A.ctl->out_rows);
+    direct_tail(out, W.st);
+  }
   ws_accumulate(A, acc.count, acc.csum);
 }
 
@@ -318,31 +321,62 @@ int run_dynamic(amj_ctx_t* ctx, Plan& P, const amj_opts_t* o, amj_result_t* res)
     capacity = (uint64_t)(est * 1.25) + 65536;
     if (o->dro_capacity_rows) capacity = o->dro_capacity_rows;
   }
+  // outputs of many rows are reserved in chunks (see DirectStage): the allocation then has room
+  // for the unused tail of every wave's last chunk, and a fill-up pass makes the rows dense
+  const uint64_t waves = (uint64_t)grid * kWavesPerBlock;
+  bool user_sized = o->dro && mat && o->dro_capacity_rows;
   for (int attempt = 0; attempt < 2; ++attempt) {
     ProbeOut out{};
-    if (mat && capacity > 0) {
-      const uint64_t need = capacity * (uint64_t)P.n_attrs * 4;
+    const uint32_t chunk = mat ? direct_chunk_rows(capacity, waves) : 0u;
+    uint64_t alloc = capacity;
+    if (chunk) {
+      alloc = capacity + (user_sized ? 0 : waves * (uint64_t)chunk);
+      alloc = (alloc + chunk - 1) / chunk * chunk;
+    }
+    if (mat && alloc > 0) {
+      const uint64_t need = alloc * (uint64_t)P.n_attrs * 4;
       if (need > budget_left(ctx, o)) {
-        set_error(ctx, AMJ_ERR_NOMEM, "result of " + std::to_string(capacity) +
+        set_error(ctx, AMJ_ERR_NOMEM, "result of " + std::to_string(alloc) +
                                           " tuples does not fit the HBM budget (use ooc=1 or "
                                           "materialize=0)");
         return AMJ_ERR_NOMEM;
       }
-      AMJ_TRY(alloc_output(ctx, capacity, P.n_attrs, res));
+      AMJ_TRY(alloc_output(ctx, alloc, P.n_attrs, res));
       for (int a = 0; a < P.n_attrs; ++a) out.cols[a] = res->cols[a];
-      out.capacity = capacity;
+      out.capacity = alloc;
+      if (chunk) {
+        void* tails = nullptr;
+        AMJ_TRY(scratch(ctx, SLOT_MISC2, alloc / chunk * 8, &tails));
+        AMJ_HIP_TRY(ctx, hipMemsetAsync(tails, 0, alloc / chunk * 8, s));
+        out.chunk = chunk;
+        out.tails = (uint64_t*)tails;
+      }
     }
-    A.out_cap = capacity;
+    A.out_cap = out.capacity;
     const bool hash = o->checksum || !mat;
     AMJ_TRY(dyn_pass(ctx, P, out, hash, A, grid, &hc, &kms));
     ++passes;
-    if (hc.out_rows <= capacity) break;
-    // the estimate was short: the pass counted every result, rerun into an exact allocation
+    if (hc.out_rows <= out.capacity) {
+      if (out.chunk && hc.out_rows > hc.count) {
+        const uint64_t used_chunks = hc.out_rows / out.chunk;
+        void *tpre = nullptr, *stmp = nullptr;
+        AMJ_TRY(scratch(ctx, SLOT_MISC3, (used_chunks + 1) * 8, &tpre));
+        AMJ_TRY(scratch(ctx, SLOT_SCAN_TEMP, scan_temp_elems(used_chunks) * 8, &stmp));
+        AMJ_HIP_TRY(ctx, exclusive_scan_u64(out.tails, (uint64_t*)tpre, used_chunks,
+                                            (uint64_t*)stmp, s));
+        AMJ_HIP_TRY(ctx, launch_chunk_fill(out, P.n_attrs, out.tails, (const uint64_t*)tpre,
+                                           used_chunks, out.chunk, hc.count, hc.out_rows, s));
+      }
+      break;
+    }
+    // the estimate was short: the pass counted every result, rerun into an allocation with
+    // room for them (and the chunk tails)
     amj_result_free(ctx, res);
-    capacity = hc.out_rows;
+    capacity = hc.count;
+    user_sized = false;
   }
-  if (hc.out_rows > capacity) {
-    set_error(ctx, AMJ_ERR_STATE, "direct output overran an exact allocation");
+  if (mat && hc.out_rows > A.out_cap) {
+    set_error(ctx, AMJ_ERR_STATE, "direct output overran an allocation sized for it");
     return AMJ_ERR_STATE;
   }
   AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[5], s));

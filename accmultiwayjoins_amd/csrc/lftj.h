@@ -101,9 +101,6 @@ hipError_t launch_lftj_direct(const LPlan& P, const LOut& cols, uint64_t capacit
                               uint32_t chunk, bool hash, unsigned long long* rows,
                               uint64_t* tails, uint64_t* block_sums, uint64_t* block_counts,
                               uint64_t* n_blocks, const uint64_t* order, hipStream_t s);
-hipError_t launch_chunk_fill(const LOut& cols, int n_attrs, const uint64_t* tail,
-                             const uint64_t* tpre, uint64_t n_chunks, uint32_t chunk, uint64_t n,
-                             uint64_t reserved, hipStream_t s);
 
 // Work sharing (lftj_ws.hip): persistent waves over the units (taken in `order`, nullptr: number
 // order) that split long candidate runs into tasks. `args` is a WsArgs (ws_queue.h) whose task

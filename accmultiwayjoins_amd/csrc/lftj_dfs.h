@@ -91,7 +91,7 @@ struct WriteSink {
 //     rows instead of one per buffer (one counter updated by every wave serialises them: about
 //     14 ns per update on this device, which bounded a whole triangle pass). Only a wave's last
 //     chunk is left partly unused; the wave records that tail (`tail_of`) and a fill-up pass
-//     moves the rows found past the result count into the tails below it (chunk_fill_kernel).
+//     moves the rows found past the result count into the tails below it (launch_chunk_fill).
 constexpr uint32_t kLftjDirectRows = 256;
 typedef __attribute__((address_space(3))) int32_t lds_i32;
 

@@ -155,14 +155,6 @@ static int sample_capacity(amj_ctx_t* ctx, const LPlan& P, uint64_t* capacity) {
   return AMJ_OK;
 }
 
-// Rows per reservation of the direct pass: the largest power of two in [256, 4096] whose
-// worst-case unused tails (one per wave) stay within 1/16 of the expected output.
-static uint32_t direct_chunk(uint64_t expected, uint64_t waves) {
-  uint32_t chunk = 256;
-  while (chunk < 4096 && waves * (uint64_t)chunk * 2 * 16 <= expected) chunk *= 2;
-  return chunk;
-}
-
 static int run_lftj_direct(amj_ctx_t* ctx, const LPlan& P, const amj_opts_t* o,
                            amj_result_t* res) {
   hipStream_t s = ctx->stream;
@@ -188,7 +180,7 @@ static int run_lftj_direct(amj_ctx_t* ctx, const LPlan& P, const amj_opts_t* o,
   uint32_t chunk = 256;
   double kms = 0.0;
   for (int attempt = 0; attempt < 2; ++attempt) {
-    chunk = direct_chunk(expected, units);
+    chunk = std::max<uint32_t>(direct_chunk_rows(expected, units), 256);
     const uint64_t want = expected + (slack ? units * (uint64_t)chunk : 0);
     capacity = (want + chunk - 1) / chunk * chunk;
     const uint64_t n_chunks = capacity / chunk;
@@ -224,7 +216,9 @@ static int run_lftj_direct(amj_ctx_t* ctx, const LPlan& P, const amj_opts_t* o,
         AMJ_TRY(scratch(ctx, SLOT_SCAN_TEMP, scan_temp_elems(used_chunks) * 8, &stmp));
         AMJ_HIP_TRY(ctx, exclusive_scan_u64((const uint64_t*)tail, (uint64_t*)tpre, used_chunks,
                                             (uint64_t*)stmp, s));
-        AMJ_HIP_TRY(ctx, launch_chunk_fill(out, P.n_attrs, (const uint64_t*)tail,
+        ProbeOut cols{};
+        for (int a = 0; a < P.n_attrs; ++a) cols.cols[a] = out.c[a];
+        AMJ_HIP_TRY(ctx, launch_chunk_fill(cols, P.n_attrs, (const uint64_t*)tail,
                                            (const uint64_t*)tpre, used_chunks, chunk, n,
                                            reserved, s));
       }

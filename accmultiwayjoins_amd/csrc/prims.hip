@@ -338,4 +338,51 @@ hipError_t radix_sort_u64(uint64_t* keys, uint64_t* alt, uint64_t n, uint32_t di
   return hipGetLastError();
 }
 
+// Fill-up after a chunked direct pass of n results whose chunks end at `reserved`: the rows at
+// [n, reserved) outside the chunks' unused tails move, in order, to the tail slots below n, so
+// the n results are dense. tail[c] is chunk c's unused tail (its last tail[c] rows) and tpre
+// its exclusive prefix sum, for the reserved / chunk chunks. Reads and writes never overlap
+// (sources lie at or past n, destinations below it).
+__global__ __launch_bounds__(kBlock) void chunk_fill_kernel(const ProbeOut out, int n_attrs,
+                                                            const uint64_t* __restrict__ tail,
+                                                            const uint64_t* __restrict__ tpre,
+                                                            uint64_t n_chunks, uint32_t chunk,
+                                                            uint64_t n, uint64_t reserved) {
+  // tail slots below n: every tail of the chunks before n's chunk, and the part of that
+  // chunk's tail below n
+  const uint64_t cn = n / chunk;
+  uint64_t below = cn < n_chunks ? tpre[cn] : tpre[n_chunks - 1] + tail[n_chunks - 1];
+  if (cn < n_chunks) {
+    const uint64_t keep = chunk - tail[cn], o = n % chunk;
+    below += o > keep ? o - keep : 0;
+  }
+  for (uint64_t p = n + (uint64_t)blockIdx.x * kBlock + threadIdx.x; p < reserved;
+       p += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t c = p / chunk, o = p % chunk;
+    if (o >= chunk - tail[c]) continue;  // p is in a tail
+    // results in [n, p): positions minus the tail slots among them
+    const uint64_t r = (p - n) - (tpre[c] - below);
+    // the r-th tail slot: chunk c2 = last chunk with tpre[c2] <= r
+    uint64_t lo = 0, hi = cn < n_chunks ? cn + 1 : n_chunks;
+    while (hi - lo > 1) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (tpre[mid] <= r) lo = mid;
+      else hi = mid;
+    }
+    const uint64_t dst = lo * chunk + (chunk - tail[lo]) + (r - tpre[lo]);
+    for (int a = 0; a < n_attrs; ++a) out.cols[a][dst] = out.cols[a][p];
+  }
+}
+
+hipError_t launch_chunk_fill(const ProbeOut& out, int n_attrs, const uint64_t* tail,
+                             const uint64_t* tpre, uint64_t n_chunks, uint32_t chunk, uint64_t n,
+                             uint64_t reserved, hipStream_t s) {
+  if (reserved <= n || n_chunks == 0 || chunk == 0) return hipSuccess;
+  uint64_t g = (reserved - n + kBlock - 1) / kBlock;
+  if (g > 65536) g = 65536;
+  hipLaunchKernelGGL(chunk_fill_kernel, dim3((unsigned)g), dim3(kBlock), 0, s, out, n_attrs,
+                     tail, tpre, n_chunks, chunk, n, reserved);
+  return hipGetLastError();
+}
+
 }  // namespace amj

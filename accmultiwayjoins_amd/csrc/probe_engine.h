@@ -132,17 +132,21 @@ __device__ __forceinline__ void emit(const Plan& P, int t, const LvN<NA>& l,
 }
 
 // Direct output with dynamic placement: a wave gathers its results in an LDS buffer of
-// kDirectRows rows and reserves room in the output with one atomic per full buffer, so the
-// output stays dense and the shared row counter sees one update per kDirectRows results (tiny
-// last-level frontiers, as in TRI and FOUR, would otherwise make one update per few results).
-// Rows past the output's capacity are not written; the reserved total then tells the host that
-// the pass must run again into an exact allocation.
+// kDirectRows rows and reserves room in the output with one atomic per full buffer (tiny
+// last-level frontiers, as in TRI and FOUR, would otherwise make one update per few results),
+// or, when out.chunk != 0, one atomic per out.chunk rows: the wave fills its chunk buffer by
+// buffer, since a row counter updated by every wave serialises the updates (about 14 ns each on
+// this device), and records the unused tail of its last chunk for the fill-up pass
+// (launch_chunk_fill) that makes the output dense. Rows past the output's capacity are not
+// written; the reserved total then tells the host that the pass must run again.
 constexpr uint32_t kDirectRows = 256;
 typedef __attribute__((address_space(3))) int32_t lds_i32;
 
 struct DirectStage {
-  lds_i32* buf;   // this wave's rows: attribute a of row i at buf[a * kDirectRows + i]
-  uint32_t fill;  // rows held (uniform)
+  lds_i32* buf;          // this wave's rows: attribute a of row i at buf[a * kDirectRows + i]
+  uint32_t fill;         // rows held (uniform)
+  uint64_t cur = ~0ull;  // chunked: first row of the current chunk (~0: none yet)
+  uint32_t used = 0;     // chunked: rows of the current chunk taken
 };
 
 // LDS of a block's staging buffers for a query of L tables.
@@ -153,14 +157,34 @@ template <int NA>
 __device__ __forceinline__ void direct_flush(const ProbeOut& out, int na, DirectStage& st,
                                              unsigned long long* rows) {
   if (st.fill == 0) return;
-  unsigned long long r0 = 0;
-  if (lane_id() == 0)
-    r0 = __hip_atomic_fetch_add(rows, (unsigned long long)st.fill, __ATOMIC_RELAXED,
-                                __HIP_MEMORY_SCOPE_AGENT);
-  const uint64_t base = uniform((uint64_t)r0);
+  // rows [0, nb) of the buffer go to base_a + i, rows [nb, fill) to base_b + i - nb
+  uint64_t base_a = 0, base_b = 0;
+  uint32_t nb = st.fill;
+  if (out.chunk) {
+    const uint32_t rem = st.cur == ~0ull ? 0u : out.chunk - st.used;
+    nb = st.fill < rem ? st.fill : rem;
+    base_a = st.cur + st.used;
+    if (st.fill > rem) {
+      unsigned long long r0 = 0;
+      if (lane_id() == 0)
+        r0 = __hip_atomic_fetch_add(rows, (unsigned long long)out.chunk, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
+      base_b = uniform((uint64_t)r0);
+      st.cur = base_b;
+      st.used = st.fill - nb;
+    } else {
+      st.used += st.fill;
+    }
+  } else {
+    unsigned long long r0 = 0;
+    if (lane_id() == 0)
+      r0 = __hip_atomic_fetch_add(rows, (unsigned long long)st.fill, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
+    base_a = uniform((uint64_t)r0);
+  }
   __asm__ __volatile__("" ::: "memory");  // LDS ops of one wave complete in order
   for (uint32_t i = (uint32_t)lane_id(); i < st.fill; i += kWave) {
-    const uint64_t pos = base + i;
+    const uint64_t pos = i < nb ? base_a + i : base_b + (i - nb);
     if (pos < out.capacity)
       sfor<0, NA>([&](auto a) {
         if (a < na) out.cols[a][pos] = st.buf[a * kDirectRows + i];
@@ -168,6 +192,13 @@ __device__ __forceinline__ void direct_flush(const ProbeOut& out, int na, Direct
   }
   __asm__ __volatile__("" ::: "memory");
   st.fill = 0;
+}
+
+// After a wave's last flush (chunked output): the unused tail of its last chunk, when the chunk
+// lies inside the output.
+__device__ __forceinline__ void direct_tail(const ProbeOut& out, const DirectStage& st) {
+  if (out.chunk && st.cur != ~0ull && lane_id() == 0 && st.cur + out.chunk <= out.capacity)
+    out.tails[st.cur / out.chunk] = out.chunk - st.used;
 }
 
 // Results [0, T) of a last level into the wave's buffer (MAT) and the checksum (HASH).

@@ -25,7 +25,7 @@ struct WsCtl {
   unsigned long long next_unit;  // units claimed
   unsigned long long q_tail;     // task slots reserved (may pass cap: clamp on read)
   unsigned long long q_head;     // task slots claimed
-  unsigned long long out_rows;   // output rows reserved (== results once the pass ends)
+  unsigned long long out_rows;   // output rows reserved (the results, plus unused chunk tails)
   unsigned long long count;      // results
   unsigned long long csum;       // checksum, sum of the waves' partials mod 2^64
   unsigned long long pushed;     // tasks published
