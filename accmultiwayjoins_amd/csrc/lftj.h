@@ -44,6 +44,8 @@ struct LPlan {
   int32_t driver0;                     // root relation whose level 0 enumerates attribute 0
   int32_t off32;                       // every trie array is addressable with 32-bit byte offsets
   uint64_t roots;                      // level-0 size of driver0
+  uint32_t rpu;                        // roots per unit of the DFS (1..64; 0: 64)
+  uint32_t pad;
 };
 
 // Query shapes with kernels of their own: relations listed by (x, y) ascending (the order the
@@ -69,7 +71,7 @@ int trie_from_host(amj_ctx_t* ctx, uint64_t n0, const int32_t* d0, const uint64_
                    uint64_t n1, const int32_t* d1, TrieDev* out);
 void free_trie(amj_ctx_t* ctx, TrieDev* t);
 
-// DFS engine (lftj.hip): per-unit counts of 64 roots, then the write pass at scanned bases.
+// DFS engine (lftj.hip): per-unit counts of P.rpu roots, then the write pass at scanned bases.
 // `order` (nullptr: units in number order) is a launch order from launch_lftj_unit_weights
 // sorted ascending: wave slot k runs the unit in the low 32 bits of order[k].
 uint64_t lftj_units(const LPlan& P);
@@ -105,11 +107,15 @@ hipError_t launch_lftj_direct(const LPlan& P, const LOut& cols, uint64_t capacit
 // Work sharing (lftj_ws.hip): persistent waves over the units (taken in `order`, nullptr: number
 // order) that split long candidate runs into tasks. `args` is a WsArgs (ws_queue.h) whose task
 // buffer holds cap records of lftj_ws_task_bytes(). mode: MODE_COUNT / MODE_HASH count (and
-// hash) into the control block; MODE_WRITE / MODE_WRITE_HASH write with direct output.
+// hash) into the control block; MODE_WRITE / MODE_WRITE_HASH write with direct output, rows
+// reserved `chunk` at a time (0: one reservation per 256-row buffer) with the unused tail of
+// every wave's last chunk in tails[] (zeroed by the caller), as launch_lftj_direct; the control
+// block's count is then the result count and out_rows the reserved rows.
 uint64_t lftj_ws_task_bytes();
 int lftj_ws_blocks_per_cu(const LPlan& P, int mode);
 hipError_t launch_lftj_ws(const LPlan& P, const LOut& cols, int mode, const void* args,
-                          const uint64_t* order, unsigned grid, hipStream_t s);
+                          const uint64_t* order, unsigned grid, hipStream_t s,
+                          uint32_t chunk = 0, uint64_t* tails = nullptr);
 
 // Level-0 arrays of a relation readable from the host.
 struct HostView {

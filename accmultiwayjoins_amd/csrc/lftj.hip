@@ -26,8 +26,8 @@ namespace amj {
 template <uint64_t SH>
 __device__ __forceinline__ lds_i32* coop_stage() {
   if constexpr (SH != 0) {
-    __shared__ int32_t coop_mem[kWavesPerBlock * kStageInts];
-    return (lds_i32*)&coop_mem[(threadIdx.x / kWave) * kStageInts];
+    __shared__ int32_t coop_mem[kWavesPerBlock * stage_ints<SH>()];
+    return (lds_i32*)&coop_mem[(threadIdx.x / kWave) * stage_ints<SH>()];
   } else {
     return nullptr;
   }
@@ -135,9 +135,9 @@ __global__ __launch_bounds__(kBlock) void lftj_direct_kernel(const LPlan P, cons
                                                              const uint64_t* __restrict__ order) {
   __shared__ int32_t stage_mem[kWavesPerBlock * A * kLftjDirectRows];
   const uint64_t k = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
-  DirectSink<A, HASH, true> sink{cols, capacity, rows,
-                                 (lds_i32*)&stage_mem[(threadIdx.x / kWave) * A * kLftjDirectRows],
-                                 0u, 0};
+  DirectSink<A, HASH> sink{cols, capacity, rows,
+                           (lds_i32*)&stage_mem[(threadIdx.x / kWave) * A * kLftjDirectRows], 0u,
+                           0};
   sink.chunk = chunk;
   if (k < units) {
     LLv<A> l0;
@@ -150,7 +150,10 @@ __global__ __launch_bounds__(kBlock) void lftj_direct_kernel(const LPlan P, cons
                    &block_counts[blockIdx.x]);
 }
 
-uint64_t lftj_units(const LPlan& P) { return (P.roots + kWave - 1) / kWave; }
+uint64_t lftj_units(const LPlan& P) {
+  const uint64_t rpu = P.rpu ? P.rpu : (uint64_t)kWave;
+  return (P.roots + rpu - 1) / rpu;
+}
 
 template <int V>
 using IntC = std::integral_constant<int, V>;

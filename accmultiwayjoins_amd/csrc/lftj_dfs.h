@@ -26,11 +26,13 @@ __device__ __forceinline__ void level0_at(const LPlan& P, uint64_t r, bool valid
   finish(l);
 }
 
-// Level 0 of a unit: 64 consecutive level-0 values of driver0.
+// Level 0 of a unit: P.rpu (0: 64) consecutive level-0 values of driver0, one per lane.
 template <int A, uint64_t SH = 0>
 __device__ __forceinline__ void level0(const LPlan& P, uint64_t unit, LLv<A>& l) {
-  const uint64_t r = unit * kWave + (uint64_t)lane_id();
-  level0_at<A, SH>(P, r, r < P.roots, l);
+  const uint32_t rpu = P.rpu ? P.rpu : (uint32_t)kWave;
+  const uint32_t lane = (uint32_t)lane_id();
+  const uint64_t r = unit * rpu + lane;
+  level0_at<A, SH>(P, r, lane < rpu && r < P.roots, l);
 }
 
 // What happens to the candidates that pass the last attribute (`ok` per lane, bindings in b):
@@ -85,19 +87,19 @@ struct WriteSink {
 // Direct output: results gather in a per-wave LDS buffer of kLftjDirectRows rows; rows at or
 // past `capacity` are not written (the reserved total then tells the host to run again with a
 // larger allocation).
-//   * CHUNKED = false: one atomic reservation on the launch's row counter per full buffer.
-//   * CHUNKED = true: rows are reserved `chunk` at a time (a multiple of kLftjDirectRows) and a
-//     wave fills its chunk buffer by buffer, so the shared counter sees one update per `chunk`
-//     rows instead of one per buffer (one counter updated by every wave serialises them: about
-//     14 ns per update on this device, which bounded a whole triangle pass). Only a wave's last
-//     chunk is left partly unused; the wave records that tail (`tail_of`) and a fill-up pass
-//     moves the rows found past the result count into the tails below it (launch_chunk_fill).
+//   * chunk = 0: one atomic reservation on the launch's row counter per full buffer.
+//   * chunk > 0: rows are reserved `chunk` at a time (a multiple of kLftjDirectRows) and a wave
+//     fills its chunk buffer by buffer, so the shared counter sees one update per `chunk` rows
+//     instead of one per buffer (one counter updated by every wave serialises them: about 14 ns
+//     per update on this device, which bounded a whole triangle pass). Only a wave's last chunk
+//     is left partly unused; the wave records that tail (`tail_of`) and a fill-up pass moves the
+//     rows found past the result count into the tails below it (launch_chunk_fill).
 constexpr uint32_t kLftjDirectRows = 256;
 typedef __attribute__((address_space(3))) int32_t lds_i32;
 
 // The checksum is taken from the buffer when it is flushed, with every lane busy, rather than
 // per result on the lanes whose candidate passed.
-template <int A, bool HASH, bool CHUNKED = false>
+template <int A, bool HASH>
 struct DirectSink {
   LOut cols;
   uint64_t capacity;
@@ -105,9 +107,9 @@ struct DirectSink {
   lds_i32* buf;              // attribute a of row i at buf[a * kLftjDirectRows + i]
   uint32_t fill;             // rows held (uniform)
   uint64_t csum;
-  uint32_t chunk = 0;        // CHUNKED: rows per reservation
-  uint64_t cur = ~0ull;      // CHUNKED: first row of the wave's current chunk (~0: none yet)
-  uint32_t used = 0;         // CHUNKED: rows of the current chunk taken
+  uint32_t chunk = 0;        // rows per reservation (0: one reservation per buffer)
+  uint64_t cur = ~0ull;      // first row of the wave's current chunk (~0: none yet)
+  uint32_t used = 0;         // rows of the current chunk taken
   uint64_t count = 0;        // rows passed to the sink (uniform)
 
   __device__ __forceinline__ void flush() {
@@ -115,7 +117,7 @@ struct DirectSink {
     // rows [0, na) of the buffer go to base_a + i, rows [na, fill) to base_b + i - na
     uint64_t base_a = 0, base_b = 0;
     uint32_t na = fill;
-    if constexpr (CHUNKED) {
+    if (chunk) {
       const uint32_t rem = cur == ~0ull ? 0u : chunk - used;
       na = fill < rem ? fill : rem;
       base_a = cur + used;
@@ -153,10 +155,10 @@ struct DirectSink {
     fill = 0;
   }
 
-  // CHUNKED, after the last flush: the unused tail of the wave's last chunk, recorded by lane 0
+  // Chunked, after the last flush: the unused tail of the wave's last chunk, recorded by lane 0
   // as tail[chunk index] when the chunk lies inside the allocation.
   __device__ __forceinline__ void tail_of(uint64_t* tail) const {
-    if (CHUNKED && cur != ~0ull && lane_id() == 0 && cur + chunk <= capacity)
+    if (chunk && cur != ~0ull && lane_id() == 0 && cur + chunk <= capacity)
       tail[cur / chunk] = chunk - used;
   }
 
@@ -212,6 +214,13 @@ __device__ __forceinline__ bool lchild(const LPlan& P, const LLv<A>& par, uint32
 // chain of dependent ones). The other entries go through the flattened enumeration.
 constexpr uint32_t kCoopMinCands = 32;
 constexpr uint32_t kStageInts = 1024;  // per wave
+// 4-clique pairs (descend_clique_pairs): the other list of an entry (at most kCliqueOther
+// values) and the entry's intersection after it, in a stage of 2 * kCliqueOther values.
+constexpr uint32_t kCliqueOther = 768;
+template <uint64_t SH>
+constexpr uint32_t stage_ints() {
+  return SH == kShapeFour ? 2 * kCliqueOther : kStageInts;
+}
 
 __device__ __forceinline__ uint32_t readlane_u32(uint32_t v, int lane) {
   return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
@@ -354,11 +363,139 @@ __device__ __forceinline__ void descend_last_fixed(const LPlan& P, const LLv<A>&
   }
 }
 
+// Attributes 2 and 3 of a 4-clique (compiled shape kShapeFour) whose relations (0,2) and (0,3)
+// are one trie, and (1,2) and (1,3) another. For an entry (a0, a1) the third attribute takes
+// L = children(a0) ∩ children'(a1), and the fourth takes the same L intersected with the children
+// of a2 in relation (2,3). An entry whose list to check holds at most kCliqueOther values is run
+// by the whole wave: L is formed once in LDS (the other list staged, the driver's candidates
+// checked there and the passing ones appended in order, so L stays sorted), then every a2 of L
+// opens its children in (2,3), and each child found in L is a result: one LDS search in the
+// short list L per candidate, instead of two searches in long lists per candidate. The other
+// entries take the generic path.
+template <int A, uint64_t SH, typename Sink>
+__device__ __forceinline__ void descend_clique_pairs(const LPlan& P, const LLv<A>& par,
+                                                     Sink& sink, lds_i32* stage) {
+  static_assert(A == 4 && SH == kShapeFour, "4-clique shape only");
+  constexpr int K = kCoopChunks;
+  // the relation checked at attribute 2 besides the driver (1: children of a0, 3: of a1)
+  const bool drv1 = par.drel == 1;
+  const int32_t* optr = drv1 ? P.rel[3].d1 + par.lo[3] : P.rel[1].d1 + par.lo[1];
+  const uint32_t olen = drv1 ? par.hi[3] - par.lo[3] : par.hi[1] - par.lo[1];
+  const bool clique = par.dlen > 0 && olen <= kCliqueOther;
+
+  // entries with a long list to check: the generic levels
+  {
+    LLv<A> sm = par;
+    sm.dlen = clique ? 0u : par.dlen;
+    finish(sm);
+    for (uint32_t c = 0; c < sm.T; c += (uint32_t)kWave) {
+      LLv<A> ch;
+      const bool ok = lchild<A, 2, SH>(P, sm, c, sm.T, ch);
+      choose_driver<A, SH>(P, 3, ok, ch);
+      finish(ch);
+      if (ch.T) descend_last_fixed<A, 3, SH>(P, ch, sink, stage);
+    }
+  }
+
+  lds_i32* lst = stage + kCliqueOther;  // L
+  uint64_t m = __ballot(clique);
+  const uint32_t lane = (uint32_t)lane_id();
+  uint64_t staged = 0;
+  uint32_t slen = 0;
+  const LRel& r5 = P.rel[5];
+  while (m) {
+    const int p = __ffsll((unsigned long long)m) - 1;
+    m &= m - 1;
+    const int32_t* dp = (const int32_t*)readlane_u64((uint64_t)par.dptr, p);
+    const uint32_t dl = readlane_u32(par.dlen, p);
+    const int32_t* src = (const int32_t*)readlane_u64((uint64_t)optr, p);
+    const uint32_t n = readlane_u32(olen, p);
+    if ((uint64_t)src != staged || n != slen) {
+      __asm__ __volatile__("" ::: "memory");  // earlier reads of the stage are done (in order)
+      for (uint32_t j0 = 0; j0 < n; j0 += 4u * kWave) {
+        int32_t t[4];
+        sfor<0, 4>([&](auto u) {
+          const uint32_t j = j0 + (uint32_t)u * kWave + lane;
+          t[u] = j < n ? ld_at<false>(src, j) : 0;
+        });
+        sfor<0, 4>([&](auto u) {
+          const uint32_t j = j0 + (uint32_t)u * kWave + lane;
+          if (j < n) stage[j] = t[u];
+        });
+      }
+      staged = (uint64_t)src;
+      slen = n;
+      __asm__ __volatile__("" ::: "memory");  // LDS ops of one wave complete in order
+    }
+    // L = the driver's candidates found in the staged list, in order
+    uint32_t nl = 0;
+    for (uint32_t c = 0; c < dl; c += (uint32_t)(K * kWave)) {
+      int32_t v[K];
+      bool ok[K];
+      sfor<0, K>([&](auto j) {
+        const uint32_t g = c + (uint32_t)j * kWave + lane;
+        ok[j] = g < dl;
+        v[j] = ok[j] ? ld_at<false>(dp, g) : 0;
+      });
+      lds_has<K>(stage, n, v, ok);
+      sfor<0, K>([&](auto j) {
+        const uint64_t bm = __ballot(ok[j]);
+        if (ok[j]) lst[nl + mbcnt(bm)] = v[j];
+        nl += (uint32_t)__popcll(bm);
+      });
+    }
+    __asm__ __volatile__("" ::: "memory");
+    if (nl == 0) continue;
+    int32_t b[A];
+    b[0] = (int32_t)readlane_u32((uint32_t)par.b[0], p);
+    b[1] = (int32_t)readlane_u32((uint32_t)par.b[1], p);
+    // every a2 of L (64 at a time) with its children in (2,3), flattened over the wave
+    for (uint32_t g0 = 0; g0 < nl; g0 += (uint32_t)kWave) {
+      const bool has = g0 + lane < nl;
+      const int32_t a2 = has ? lst[g0 + lane] : 0;
+      uint32_t clo = 0, chi = 0;
+      const bool f = has && root_range<true>(r5, a2, clo, chi);
+      const uint32_t len = f ? chi - clo : 0u;
+      const uint32_t incl = wave_scan_add_dpp(len);
+      const uint32_t excl = incl - len;
+      const uint32_t T = lane63(incl);
+      for (uint32_t c = 0; c < T; c += (uint32_t)kWave) {
+        const uint32_t g = c + lane;
+        const int q = (int)parent_of_slots(excl, len, c);
+        const uint32_t k = g - (uint32_t)__shfl(excl, q, kWave);
+        const uint32_t qlo = (uint32_t)__shfl(clo, q, kWave);
+        bool ok[1] = {g < T};
+        int32_t a3[1] = {ok[0] ? ld_at<false>(r5.d1, qlo + k) : 0};
+        lds_has<1>(lst, nl, a3, ok);
+        b[2] = __shfl(a2, q, kWave);
+        b[3] = a3[0];
+        sink(ok[0], b);
+      }
+    }
+  }
+}
+
 template <int A, int i, uint64_t SH = 0, typename Sink>
 __device__ __forceinline__ void descend(const LPlan& P, const LLv<A>& par, Sink& sink,
                                         lds_i32* stage = nullptr) {
   if constexpr (SH != 0 && i == A - 1) {
     descend_last_fixed<A, i, SH>(P, par, sink, stage);
+  } else if constexpr (SH == kShapeFour && i == 2) {
+    // (0,2) and (0,3) one trie, (1,2) and (1,3) another: the clique pairs apply
+    const bool same = P.rel[1].d1 == P.rel[2].d1 && P.rel[1].off == P.rel[2].off &&
+                      P.rel[1].d0 == P.rel[2].d0 && P.rel[3].d1 == P.rel[4].d1 &&
+                      P.rel[3].off == P.rel[4].off && P.rel[3].d0 == P.rel[4].d0;
+    if (same) {
+      descend_clique_pairs<A, SH>(P, par, sink, stage);
+      return;
+    }
+    for (uint32_t c = 0; c < par.T; c += (uint32_t)kWave) {
+      LLv<A> ch;
+      const bool ok = lchild<A, i, SH>(P, par, c, par.T, ch);
+      choose_driver<A, SH>(P, i + 1, ok, ch);
+      finish(ch);
+      if (ch.T) descend<A, i + 1, SH>(P, ch, sink, stage);
+    }
   } else {
     for (uint32_t c = 0; c < par.T; c += (uint32_t)kWave) {
       LLv<A> ch;

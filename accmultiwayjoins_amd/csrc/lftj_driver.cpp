@@ -286,12 +286,13 @@ constexpr uint64_t kLftjStallMs = 20000;     // no progress for this long: give 
 
 static int lftj_ws_pass(amj_ctx_t* ctx, const LPlan& P, const LOut& out, int mode,
                         const WsArgs& A, const uint64_t* order, unsigned grid, WsCtl* hc,
-                        double* kms) {
+                        double* kms, uint32_t chunk = 0, uint64_t* tails = nullptr) {
   hipStream_t s = ctx->stream;
   AMJ_HIP_TRY(ctx, hipMemsetAsync(A.ctl, 0, sizeof(WsCtl), s));
   if (A.split) AMJ_HIP_TRY(ctx, hipMemsetAsync(A.ready, 0, A.cap * 4, s));
   AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[3], s));
-  if (A.units) AMJ_HIP_TRY(ctx, launch_lftj_ws(P, out, mode, &A, order, grid, s));
+  if (A.units)
+    AMJ_HIP_TRY(ctx, launch_lftj_ws(P, out, mode, &A, order, grid, s, chunk, tails));
   AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[4], s));
   AMJ_HIP_TRY(ctx, hipMemcpyAsync(hc, A.ctl, sizeof(WsCtl), hipMemcpyDeviceToHost, s));
   AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
@@ -303,8 +304,13 @@ static int lftj_ws_pass(amj_ctx_t* ctx, const LPlan& P, const LOut& out, int mod
   return AMJ_OK;
 }
 
-static int run_lftj_ws(amj_ctx_t* ctx, const LPlan& P, const amj_opts_t* o, amj_result_t* res) {
+static int run_lftj_ws(amj_ctx_t* ctx, const LPlan& P_in, const amj_opts_t* o,
+                       amj_result_t* res) {
   hipStream_t s = ctx->stream;
+  // units of 64 roots: the waves claim them dynamically and split long runs into tasks, so
+  // they need not be small
+  LPlan P = P_in;
+  P.rpu = kWave;
   const uint64_t units = lftj_units(P);
   WsArgs A{};
   void *ctl = nullptr, *tasks = nullptr, *ready = nullptr;
@@ -355,34 +361,68 @@ static int run_lftj_ws(amj_ctx_t* ctx, const LPlan& P, const amj_opts_t* o, amj_
       pushed += hc.pushed;
       capacity = hc.count;
     }
+    // large outputs are reserved in chunks: room for the unused tail of every wave's last
+    // chunk, then the fill-up pass (as in run_lftj_direct)
+    const uint64_t waves = (uint64_t)grid * kWavesPerBlock;
+    bool user_sized = o->dro_capacity_rows != 0;
     uint64_t rows = 0;
     for (int attempt = 0; attempt < 2; ++attempt) {
       LOut out{};
-      if (capacity > 0) {
-        if (capacity * (uint64_t)P.n_attrs * 4 > budget_left(ctx, o)) {
-          set_error(ctx, AMJ_ERR_NOMEM, "result of " + std::to_string(capacity) +
+      const uint32_t chunk = direct_chunk_rows(capacity, waves);
+      uint64_t alloc = capacity;
+      if (chunk) {
+        alloc = capacity + (user_sized ? 0 : waves * (uint64_t)chunk);
+        alloc = (alloc + chunk - 1) / chunk * chunk;
+      }
+      void* tails = nullptr;
+      if (alloc > 0) {
+        if (alloc * (uint64_t)P.n_attrs * 4 > budget_left(ctx, o)) {
+          set_error(ctx, AMJ_ERR_NOMEM, "result of " + std::to_string(alloc) +
                                             " tuples does not fit the HBM budget (use "
                                             "materialize=0)");
           return AMJ_ERR_NOMEM;
         }
-        AMJ_TRY(alloc_output(ctx, capacity, P.n_attrs, res));
+        AMJ_TRY(alloc_output(ctx, alloc, P.n_attrs, res));
         for (int a = 0; a < P.n_attrs; ++a) out.c[a] = res->cols[a];
+        if (chunk) {
+          AMJ_TRY(scratch(ctx, SLOT_MISC2, alloc / chunk * 8, &tails));
+          AMJ_HIP_TRY(ctx, hipMemsetAsync(tails, 0, alloc / chunk * 8, s));
+        }
       }
-      A.out_cap = capacity;
-      AMJ_TRY(lftj_ws_pass(ctx, P, out, final_mode, A, order, grid, &hc, &kms));
+      A.out_cap = alloc;
+      AMJ_TRY(lftj_ws_pass(ctx, P, out, final_mode, A, order, grid, &hc, &kms,
+                           tails ? chunk : 0u, (uint64_t*)tails));
       ++passes;
       pushed += hc.pushed;
       rows = hc.out_rows;
-      if (rows <= capacity) break;
-      amj_result_free(ctx, res);  // overran: the pass counted every result, run again exactly
-      capacity = rows;
+      if (rows <= alloc) {
+        if (tails && rows > hc.count) {
+          const uint64_t used_chunks = rows / chunk;
+          void *tpre = nullptr, *stmp = nullptr;
+          AMJ_TRY(scratch(ctx, SLOT_MISC3, (used_chunks + 1) * 8, &tpre));
+          AMJ_TRY(scratch(ctx, SLOT_SCAN_TEMP, scan_temp_elems(used_chunks) * 8, &stmp));
+          AMJ_HIP_TRY(ctx, exclusive_scan_u64((const uint64_t*)tails, (uint64_t*)tpre,
+                                              used_chunks, (uint64_t*)stmp, s));
+          ProbeOut cols{};
+          for (int a = 0; a < P.n_attrs; ++a) cols.cols[a] = out.c[a];
+          AMJ_HIP_TRY(ctx, launch_chunk_fill(cols, P.n_attrs, (const uint64_t*)tails,
+                                             (const uint64_t*)tpre, used_chunks, chunk,
+                                             hc.count, rows, s));
+          AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
+        }
+        break;
+      }
+      // overran: the pass counted every result; run again with room for them
+      amj_result_free(ctx, res);
+      capacity = hc.count;
+      user_sized = false;
     }
-    if (rows > capacity) {
-      set_error(ctx, AMJ_ERR_STATE, "direct output overran an exact allocation");
+    if (rows > A.out_cap) {
+      set_error(ctx, AMJ_ERR_STATE, "direct output overran an allocation sized for it");
       return AMJ_ERR_STATE;
     }
-    res->count = rows;
-    res->rows_written = rows;
+    res->count = hc.count;
+    res->rows_written = hc.count;
     res->checksum = hash ? hc.csum : 0;
   }
   AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[5], s));
@@ -505,7 +545,15 @@ int lftj_execute(amj_ctx_t* ctx, const std::vector<LUse>& uses_in, const uint32_
     }
     return run_lftj_ooc(ctx, P, hv.data(), o, parts, res, nullptr);
   }
-  if (algo == ALFTJ_DFS) return run_lftj_dfs(ctx, P, o, res);
+  if (algo == ALFTJ_DFS) {
+    // the unit engine runs one wave per unit: with few roots, fewer roots per unit, so that a
+    // launch has many more units than the device has wave slots (a unit of 64 roots of a
+    // skewed graph can run far longer than the average one)
+    P.rpu = kWave;
+    const uint64_t target = (uint64_t)ctx->props.multiProcessorCount * 128;
+    while (P.rpu > 1 && P.roots / P.rpu < target) P.rpu >>= 1;
+    return run_lftj_dfs(ctx, P, o, res);
+  }
   // BFS: count and checksum (LFTJ_BFS::evaluate reports a count only, LFTJ_BFS.cuh:457-500)
   AMJ_HIP_TRY(ctx, hipEventRecord(ctx->ev[2], ctx->stream));
   AMJ_TRY(run_lftj_bfs(ctx, P, budget_left(ctx, o), &res->count, &res->checksum));

@@ -14,13 +14,18 @@
 // keeps the next `split` and publishes the rest as tasks of `split` candidates each: the task is
 // the level's 64-lane frontier (bindings, every relation's child range, the candidate list of
 // each entry and its prefix) plus the candidate range, so the wave that takes it checks exactly
-// the candidates the owner would have. Every level can be split, the last one included: the
-// last attribute's candidates still have to be intersected (for a triangle that is where the
-// time goes). Each candidate is checked against the other lists with a search over that
-// list's whole child range rather than a galloping-bounded sub-range. The queue, the ready
-// flags, the unit claims and the termination are the protocol of the persistent AMHJ engine
-// (ws_queue.h); results are counted and hashed, or written through per-wave LDS buffers with
-// one atomic reservation per 256 rows (direct output).
+// the candidates the owner would have. For a query of no compiled shape every level can be
+// split, the last one included (the last attribute's candidates still have to be intersected).
+// For a triangle or a 4-clique (compiled shapes, lftj_common.h) the levels that the unit engine
+// runs cooperatively (the last attribute; for a 4-clique the last two) take the same
+// cooperative paths here on the task's candidate range, and the levels above them are split.
+// Each candidate is checked against the other lists with a search over that list's whole child
+// range rather than a galloping-bounded sub-range. The queue, the ready flags, the unit claims
+// and the termination are the protocol of the persistent AMHJ engine (ws_queue.h); results are
+// counted and hashed, or written through per-wave LDS buffers with chunked reservations (direct
+// output, DirectSink).
+#include <type_traits>
+
 #include "lftj_dfs.h"
 #include "ws_queue.h"
 
@@ -89,11 +94,35 @@ __device__ __forceinline__ void read_ltask(const WsArgs& W, uint64_t slot, int& 
   l.excl = lw[kLwExcl * kWave + lane];
 }
 
+// The entries of `par` cut to candidates [c0, c1) of the flattened level.
+template <int A>
+__device__ __forceinline__ void restrict_range(LLv<A>& l, uint32_t c0, uint32_t c1) {
+  const uint32_t b = l.excl > c0 ? l.excl : c0;
+  const uint32_t e = l.excl + l.dlen < c1 ? l.excl + l.dlen : c1;
+  const uint32_t n = e > b ? e - b : 0u;
+  l.dptr = n ? l.dptr + (b - l.excl) : l.dptr;
+  l.dlen = n;
+  finish(l);
+}
+
+// Levels that the unit engine runs cooperatively for a compiled shape.
+template <int A, int i, uint64_t SH>
+constexpr bool coop_level() {
+  return SH != 0 && (i == A - 1 || (SH == kShapeFour && i == 2));
+}
+
 // Candidates [c0, c1) of level `par` for attribute i, sharing the tail when it is long.
-template <int A, int i, typename Sink>
+template <int A, int i, uint64_t SH, typename Sink>
 __device__ __forceinline__ void ws_ldescend(const LPlan& P, const LLv<A>& par, uint32_t c0,
                                             uint32_t c1, Sink& sink, const WsArgs& W,
-                                            WsWave& q) {
+                                            WsWave& q, lds_i32* stage) {
+  if constexpr (coop_level<A, i, SH>()) {
+    ws_step(W, q);
+    LLv<A> r = par;
+    restrict_range(r, c0, c1);
+    if (r.T) descend<A, i, SH>(P, r, sink, stage);
+    return;
+  }
   uint64_t end = c1;
   for (uint64_t c = c0; c < end; c += kWave) {
     ws_step(W, q);
@@ -114,27 +143,33 @@ __device__ __forceinline__ void ws_ldescend(const LPlan& P, const LLv<A>& par, u
       }
     }
     LLv<A> ch;
-    const bool ok = lchild<A, i>(P, par, (uint32_t)c, (uint32_t)end, ch);
+    const bool ok = lchild<A, i, SH>(P, par, (uint32_t)c, (uint32_t)end, ch);
     if constexpr (i == A - 1) {
       sink(ok, ch.b);
     } else {
-      choose_driver(P, i + 1, ok, ch);
+      choose_driver<A, SH>(P, i + 1, ok, ch);
       finish(ch);
-      if (ch.T) ws_ldescend<A, i + 1>(P, ch, 0u, ch.T, sink, W, q);
+      if (ch.T) ws_ldescend<A, i + 1, SH>(P, ch, 0u, ch.T, sink, W, q, stage);
     }
   }
 }
 
 // One persistent pass: MODE_COUNT / MODE_HASH count (and hash) the results, MODE_WRITE /
 // MODE_WRITE_HASH write them with direct output into cols (W.out_cap rows, reserved through
-// ctl->out_rows). Units are taken in `order` (nullptr: number order).
-template <int A, int MODE>
+// ctl->out_rows `chunk` rows at a time, unused chunk tails recorded in tails; chunk 0: one
+// reservation per buffer). Units are taken in `order` (nullptr: number order).
+template <int A, uint64_t SH, int MODE>
 __global__ __launch_bounds__(kBlock) void lftj_ws_kernel(const LPlan P, const LOut cols,
                                                          const WsArgs W,
-                                                         const uint64_t* __restrict__ order) {
+                                                         const uint64_t* __restrict__ order,
+                                                         uint32_t chunk,
+                                                         uint64_t* __restrict__ tails) {
   constexpr bool MAT = mode_mat<MODE>();
   constexpr bool HASH = mode_hash<MODE>();
   __shared__ int32_t stage_mem[MAT ? kWavesPerBlock * A * kLftjDirectRows : 1];
+  __shared__ int32_t coop_mem[SH != 0 ? kWavesPerBlock * stage_ints<SH>() : 1];
+  lds_i32* stage = SH != 0 ? (lds_i32*)&coop_mem[(threadIdx.x / kWave) * stage_ints<SH>()]
+                           : nullptr;
   using Sink = std::conditional_t<MAT, DirectSink<A, HASH>, CountHashSink<HASH>>;
   Sink sink{};
   if constexpr (MAT) {
@@ -144,6 +179,7 @@ __global__ __launch_bounds__(kBlock) void lftj_ws_kernel(const LPlan P, const LO
     sink.buf = (lds_i32*)&stage_mem[(threadIdx.x / kWave) * A * kLftjDirectRows];
     sink.fill = 0u;
     sink.csum = 0;
+    sink.chunk = chunk;
   }
   WsWave q{false, 0u};
   ws_serve(
@@ -151,8 +187,8 @@ __global__ __launch_bounds__(kBlock) void lftj_ws_kernel(const LPlan P, const LO
       [&](uint64_t u, uint64_t ue) {
         for (uint64_t k = u; k < ue; ++k) {
           LLv<A> l0;
-          level0(P, unit_of(order, k), l0);
-          if (l0.T) ws_ldescend<A, 1>(P, l0, 0u, l0.T, sink, W, q);
+          level0<A, SH>(P, unit_of(order, k), l0);
+          if (l0.T) ws_ldescend<A, 1, SH>(P, l0, 0u, l0.T, sink, W, q, stage);
         }
       },
       [&](uint64_t slot) {
@@ -161,64 +197,74 @@ __global__ __launch_bounds__(kBlock) void lftj_ws_kernel(const LPlan P, const LO
         LLv<A> l;
         read_ltask<A>(W, slot, level, c0, c1, l);
         sfor<1, A>([&](auto ii) {
-          if ((int)ii == level) ws_ldescend<A, ii>(P, l, c0, c1, sink, W, q);
+          if ((int)ii == level) ws_ldescend<A, ii, SH>(P, l, c0, c1, sink, W, q, stage);
         });
       });
   if constexpr (MAT) {
     sink.flush();
-    ws_accumulate(W, 0, sink.csum);  // the count is the reserved total
-  } else {
-    ws_accumulate(W, sink.count, sink.csum);
+    sink.tail_of(tails);
   }
+  ws_accumulate(W, sink.count, sink.csum);
 }
 
-template <int A, int MODE>
+template <int A, uint64_t SH, int MODE>
 static int ws_occupancy() {
   int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lftj_ws_kernel<A, MODE>, kBlock, 0) !=
-          hipSuccess || per_cu <= 0)
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, lftj_ws_kernel<A, SH, MODE>, kBlock,
+                                                   0) != hipSuccess ||
+      per_cu <= 0)
     per_cu = 2;
   return per_cu;
 }
 
-int lftj_ws_blocks_per_cu(const LPlan& P, int mode) {
-#define OCC(AV)                                                                      \
-  switch (mode) {                                                                    \
-    case MODE_COUNT: return ws_occupancy<AV, MODE_COUNT>();                          \
-    case MODE_HASH: return ws_occupancy<AV, MODE_HASH>();                            \
-    case MODE_WRITE: return ws_occupancy<AV, MODE_WRITE>();                          \
-    default: return ws_occupancy<AV, MODE_WRITE_HASH>();                             \
-  }
+template <int V>
+using IntC = std::integral_constant<int, V>;
+template <uint64_t V>
+using ShapeC = std::integral_constant<uint64_t, V>;
+
+// f(IntC<A>, ShapeC<SH>, IntC<MODE>) for the plan's attribute count and compiled shape.
+template <typename F>
+static bool ws_dispatch(const LPlan& P, int mode, F&& f) {
+  const uint64_t sh = lftj_static_shape(P);
+  auto with_mode = [&](auto a, auto shc) {
+    switch (mode) {
+      case MODE_COUNT: f(a, shc, IntC<MODE_COUNT>{}); return true;
+      case MODE_HASH: f(a, shc, IntC<MODE_HASH>{}); return true;
+      case MODE_WRITE: f(a, shc, IntC<MODE_WRITE>{}); return true;
+      case MODE_WRITE_HASH: f(a, shc, IntC<MODE_WRITE_HASH>{}); return true;
+      default: return false;
+    }
+  };
   switch (P.n_attrs) {
-    case 2: OCC(2);
-    case 3: OCC(3);
-    case 4: OCC(4);
-    default: return 2;
+    case 2: return with_mode(IntC<2>{}, ShapeC<0>{});
+    case 3:
+      return sh == kShapeTri ? with_mode(IntC<3>{}, ShapeC<kShapeTri>{})
+                             : with_mode(IntC<3>{}, ShapeC<0>{});
+    case 4:
+      return sh == kShapeFour ? with_mode(IntC<4>{}, ShapeC<kShapeFour>{})
+                              : with_mode(IntC<4>{}, ShapeC<0>{});
+    default: return false;
   }
-#undef OCC
+}
+
+int lftj_ws_blocks_per_cu(const LPlan& P, int mode) {
+  int per_cu = 2;
+  ws_dispatch(P, mode, [&](auto a, auto sh, auto m) {
+    per_cu = ws_occupancy<decltype(a)::value, decltype(sh)::value, decltype(m)::value>();
+  });
+  return per_cu;
 }
 
 hipError_t launch_lftj_ws(const LPlan& P, const LOut& cols, int mode, const void* args,
-                          const uint64_t* order, unsigned grid, hipStream_t s) {
+                          const uint64_t* order, unsigned grid, hipStream_t s, uint32_t chunk,
+                          uint64_t* tails) {
   const WsArgs& W = *(const WsArgs*)args;
-#define LWS(AV, M) \
-  hipLaunchKernelGGL((lftj_ws_kernel<AV, M>), dim3(grid), dim3(kBlock), 0, s, P, cols, W, order)
-#define LWSA(AV)                                            \
-  switch (mode) {                                           \
-    case MODE_COUNT: LWS(AV, MODE_COUNT); break;            \
-    case MODE_HASH: LWS(AV, MODE_HASH); break;              \
-    case MODE_WRITE: LWS(AV, MODE_WRITE); break;            \
-    case MODE_WRITE_HASH: LWS(AV, MODE_WRITE_HASH); break;  \
-    default: return hipErrorInvalidValue;                   \
-  }
-  switch (P.n_attrs) {
-    case 2: LWSA(2); break;
-    case 3: LWSA(3); break;
-    case 4: LWSA(4); break;
-    default: return hipErrorInvalidValue;
-  }
-#undef LWSA
-#undef LWS
+  const bool ok = ws_dispatch(P, mode, [&](auto a, auto sh, auto m) {
+    hipLaunchKernelGGL(
+        (lftj_ws_kernel<decltype(a)::value, decltype(sh)::value, decltype(m)::value>),
+        dim3(grid), dim3(kBlock), 0, s, P, cols, W, order, chunk, tails);
+  });
+  if (!ok) return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
