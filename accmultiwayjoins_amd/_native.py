@@ -160,6 +160,8 @@ SIGNATURES = [
     ("amj_server_set_autotune", C.c_int, [C.c_void_p, C.c_uint64]),
     ("amj_server_tuning", C.c_int,
      [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
+    ("amj_server_set_triejoin", C.c_int, [C.c_void_p, C.c_uint64]),
+    ("amj_server_triejoin_runs", C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
     ("amj_server_memory", C.c_int,
      [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     ("amj_server_release", C.c_int, [C.c_void_p, C.POINTER(AmjResult)]),

@@ -48,6 +48,11 @@ struct amj_server {
   uint64_t tune_rows = 0;
   std::map<std::vector<uint64_t>, amj_opts_t> tuned_opts;
   uint64_t tuned_alone = 0;  // under mu
+  // Batched mode: TRI / FOUR requests over an edge relation of at least trie_edges rows run
+  // alone through the leapfrog triejoin, over a trie built once per relation and kept resident
+  // (under gpu); 0 = every pattern request goes into the batch.
+  uint64_t trie_edges = 0;
+  uint64_t trie_alone = 0;  // under mu
 
   std::mutex mu, gpu;
   std::condition_variable cv_work, cv_done;
@@ -57,6 +62,7 @@ struct amj_server {
   struct Resident {
     amj_relation_t* rel = nullptr;
     amj_hash_table_t* ht = nullptr;
+    amj_trie_t* trie = nullptr;  // built on the first triejoin request over the relation
   };
   std::vector<Resident> rels;          // under gpu
   std::atomic<uint32_t> n_rels{0};
@@ -112,7 +118,7 @@ struct amj_server {
     std::vector<Req> deferred;  // fit the budget, not the room left now: run after releases
     uint64_t deferred_need = 0; // bytes the first deferred request needs
     uint64_t room;
-    uint64_t batches = 0, splits = 0, tuned = 0;
+    uint64_t batches = 0, splits = 0, tuned = 0, tries = 0;
   };
 
   static Outcome failed(const Req& q, int rc, const std::string& msg) {
@@ -157,6 +163,23 @@ struct amj_server {
         if (q.query == AMJ_QUERY_NORMAL && rels[q.ids[0]].rel->rows >= tune_rows) {
           if (!c.deferred.empty()) c.deferred.push_back(q);
           else run_one(c, q, true);
+        } else {
+          small.push_back(q);
+        }
+      }
+      reqs.swap(small);
+      if (reqs.empty()) return;
+      if (!c.deferred.empty()) {
+        c.deferred.insert(c.deferred.end(), reqs.begin(), reqs.end());
+        return;
+      }
+    }
+    if (trie_edges) {  // large graph patterns alone, through the triejoin
+      std::vector<Req> small;
+      for (auto& q : reqs) {
+        if (q.query != AMJ_QUERY_NORMAL && rels[q.ids[0]].rel->rows >= trie_edges) {
+          if (!c.deferred.empty()) c.deferred.push_back(q);
+          else run_trie(c, q);
         } else {
           small.push_back(q);
         }
@@ -232,6 +255,43 @@ struct amj_server {
       return;
     }
     ++c.batches;
+    if (rc != AMJ_OK) c.out.push_back(failed(q, rc, ctx->last_error));
+    else finished(c, q, res);
+  }
+
+  // A TRI / FOUR request alone through the leapfrog triejoin (alftj_run_aliased over the edge
+  // relation's trie, aliased as in exec-LFTJ's SELF mode): a triangle with the unit engine
+  // (direct output when materialised), a 4-clique counted with work sharing. Count and checksum
+  // are those of the hash-join pattern engine (the same result set).
+  void run_trie(Cycle& c, Req q) {
+    q.t_start = now_ms();
+    Resident& r = rels[q.ids[0]];
+    int rc = AMJ_OK;
+    if (!r.trie) rc = alftj_trie_build_resident(ctx, r.rel, &r.trie);
+    amj_result_t res;
+    if (rc == AMJ_OK) {
+      amj_opts_t o;
+      amj_default_opts(&o);
+      o.materialize = q.materialize;
+      o.checksum = 1;
+      o.dro = q.materialize ? 1 : 0;
+      o.ws = q.query == AMJ_QUERY_FOUR && !q.materialize ? 1 : 0;
+      o.hbm_budget_bytes = ctx->allocated + c.room;
+      static const uint32_t tri[6] = {0, 1, 1, 2, 0, 2};
+      static const uint32_t four[12] = {0, 1, 1, 2, 2, 3, 0, 3, 0, 2, 1, 3};
+      static const uint32_t order[4] = {0, 1, 2, 3};
+      const bool is_tri = q.query == AMJ_QUERY_TRI;
+      amj_trie_t* ts[6] = {r.trie, r.trie, r.trie, r.trie, r.trie, r.trie};
+      rc = alftj_run_aliased(ctx, ts, is_tri ? 3 : 6, is_tri ? tri : four, order, is_tri ? 3 : 4,
+                             ALFTJ_DFS, &o, &res);
+    }
+    if (rc == AMJ_ERR_NOMEM && c.room < budget) {
+      c.deferred_need = budget;
+      c.deferred.push_back(q);
+      return;
+    }
+    ++c.batches;
+    ++c.tries;
     if (rc != AMJ_OK) c.out.push_back(failed(q, rc, ctx->last_error));
     else finished(c, q, res);
   }
@@ -340,6 +400,7 @@ struct amj_server {
       batches += c.batches;
       splits += c.splits;
       tuned_alone += c.tuned;
+      trie_alone += c.tries;
       for (auto& o : c.out) {
         if (o.d.status == AMJ_OK) {
           in_use += o.d.bytes;
@@ -405,6 +466,7 @@ void amj_server_destroy(amj_server_t* s) {
     if (kv.second.status == AMJ_OK) amj_result_free(s->ctx, &kv.second.res);
   for (auto& r : s->released) amj_result_free(s->ctx, &r);
   for (auto& r : s->rels) {
+    if (r.trie) alftj_trie_destroy(s->ctx, r.trie);
     amj_hash_table_destroy(s->ctx, r.ht);
     amj_relation_destroy(s->ctx, r.rel);
   }
@@ -473,6 +535,20 @@ int amj_server_set_autotune(amj_server_t* s, uint64_t min_probe_rows) {
   if (!s) return AMJ_ERR_ARG;
   std::lock_guard<std::mutex> g(s->gpu);
   s->tune_rows = min_probe_rows;
+  return AMJ_OK;
+}
+
+int amj_server_set_triejoin(amj_server_t* s, uint64_t min_edges) {
+  if (!s) return AMJ_ERR_ARG;
+  std::lock_guard<std::mutex> g(s->gpu);
+  s->trie_edges = min_edges;
+  return AMJ_OK;
+}
+
+int amj_server_triejoin_runs(amj_server_t* s, uint64_t* runs) {
+  if (!s || !runs) return AMJ_ERR_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  *runs = s->trie_alone;
   return AMJ_OK;
 }
 

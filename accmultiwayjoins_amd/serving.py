@@ -119,10 +119,16 @@ class Server:
         """Chains probing at least this many rows run alone with autotuned engine options."""
         self._check(N.lib().amj_server_set_autotune(self.h, int(min_probe_rows)))
 
+    def set_triejoin(self, min_edges: int):
+        """TRI / FOUR over edge relations of at least this many rows run alone through the
+        leapfrog triejoin (a trie built once per relation and kept resident); 0: off."""
+        self._check(N.lib().amj_server_set_triejoin(self.h, int(min_edges)))
+
     def tuning(self) -> dict:
-        a, k = C.c_uint64(), C.c_uint64()
+        a, k, t = C.c_uint64(), C.c_uint64(), C.c_uint64()
         self._check(N.lib().amj_server_tuning(self.h, C.byref(a), C.byref(k)))
-        return {"tuned_alone": a.value, "classes": k.value}
+        self._check(N.lib().amj_server_triejoin_runs(self.h, C.byref(t)))
+        return {"tuned_alone": a.value, "classes": k.value, "triejoin_alone": t.value}
 
     def set_budget(self, hbm_budget_bytes: int):
         """Result budget, before the first request (the arena is reserved at the first one)."""

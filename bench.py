@@ -333,7 +333,7 @@ def four_leg(G, scale):
             "verified": "AMHJ and ALFTJ DFS agree (count + checksum)"}
 
 
-def _serve(mode, sources, stream, gaps, budget, fit_budget=False, tune_rows=0):
+def _serve(mode, sources, stream, gaps, budget, fit_budget=False, tune_rows=0, trie_edges=0):
     """Serve `stream` through one amj_server: mode "served" (continuous batching), "plain" (one
     query per launch, count + write) or "plain-dro" (one query per launch, direct output).
     sources: relations (host arrays or DeviceColumns) registered once; stream: (kind, rel
@@ -355,6 +355,8 @@ def _serve(mode, sources, stream, gaps, budget, fit_budget=False, tune_rows=0):
         mem = srv.memory()
         if tune_rows and mode == "served":
             srv.set_autotune(tune_rows)
+        if trie_edges and mode == "served":
+            srv.set_triejoin(trie_edges)
         result_budget = budget
         if fit_budget:
             result_budget = max(1 << 30, min(budget - mem["resident_bytes"],
@@ -403,7 +405,8 @@ def _serve(mode, sources, stream, gaps, budget, fit_budget=False, tune_rows=0):
         st = srv.stats()
         tun = srv.tuning()
         log(f"[bench] serving ({mode}): {n} queries in {el:.2f} s, {st['batches']} launches "
-            f"or batches, {tun['tuned_alone']} run alone with tuned options")
+            f"or batches, {tun['tuned_alone']} run alone with tuned options, "
+            f"{tun['triejoin_alone']} through the triejoin")
     srt = sorted(lat)
     qs = sorted(qw)
     pick = lambda a, q: a[min(len(a) - 1, int(len(a) * q))]  # noqa: E731
@@ -414,11 +417,12 @@ def _serve(mode, sources, stream, gaps, budget, fit_budget=False, tune_rows=0):
             "resident_gb": round(mem["resident_bytes"] / 1e9, 2),
             "result_budget_gb": round(result_budget / 1e9, 2),
             "tuned_alone": tun["tuned_alone"], "tuned_classes": tun["classes"],
+            "triejoin_alone": tun["triejoin_alone"],
             "busy_s": round(st["busy_ms"] / 1e3, 3)}, outs
 
 
 def serving_config5_leg(n_queries, rate, seed=42, budget=200 << 30, scales=(14, 20),
-                        max_rows=50_000_000, tune_rows=4 << 20):
+                        max_rows=50_000_000, tune_rows=4 << 20, trie_edges=1 << 20):
     """The configured mixed stream: query i (seed 42 + i) is a chain of 3, 4 or 5 relations
     (60%, in equal thirds), a triangle query (20%) or a 4-clique query (20%). Chain relations
     have log-uniform sizes in [10^4, max_rows]; each join is foreign-key style: the build side's
@@ -487,7 +491,7 @@ def serving_config5_leg(n_queries, rate, seed=42, budget=200 << 30, scales=(14, 
     results, legs = {}, {}
     for mode in ("served", "plain", "plain-dro"):
         legs[mode], results[mode] = _serve(mode, sources, stream, gaps, budget, fit_budget=True,
-                                           tune_rows=tune_rows)
+                                           tune_rows=tune_rows, trie_edges=trie_edges)
     if not (results["served"] == results["plain"] == results["plain-dro"]):
         raise SystemExit("bench: served and plain results differ on the configured stream")
     for q, want in oracle_checked.items():
@@ -511,8 +515,11 @@ def serving_config5_leg(n_queries, rate, seed=42, budget=200 << 30, scales=(14, 
             "engines": "served: continuous batching (each cycle runs what arrived during the "
                        "previous one as one batch; one direct-output launch per depth and "
                        f"kind), chains probing >= {tune_rows} rows alone with the engine options "
-                       "the autotuner picks for their class; plain: one query per launch, count "
-                       "+ scan + write; plain_dro: one query per launch with direct output",
+                       "the autotuner picks for their class, TRI / FOUR over graphs of >= "
+                       f"{trie_edges} edges alone through the leapfrog triejoin (trie built once "
+                       "per graph and kept resident); plain: one query per launch of the hash-"
+                       "join engines, count + scan + write; plain_dro: one query per launch with "
+                       "direct output",
             "distinct_queries": n_queries,
             "generation_s": round(gen_s, 2),
             "verified": "every result (count + checksum) equal in the three runs; "

@@ -336,6 +336,14 @@ int amj_server_wait_ex(amj_server_t* server, uint64_t ticket, amj_result_t* res,
 int amj_server_set_autotune(amj_server_t* server, uint64_t min_probe_rows);
 /* Requests run alone with tuned options, and the classes tuned so far. */
 int amj_server_tuning(amj_server_t* server, uint64_t* tuned_alone, uint64_t* classes);
+/* Batched mode (no counterpart in the reference): TRI / FOUR requests over an edge relation of
+ * at least min_edges rows run alone through the leapfrog triejoin (exec-LFTJ's SELF aliasing of
+ * one trie, built on the first such request and kept resident with the relation): triangles by
+ * the DFS unit engine (direct output when materialised), 4-cliques counted with work sharing.
+ * Count and checksum equal the hash-join pattern engine's. 0 (default): off. */
+int amj_server_set_triejoin(amj_server_t* server, uint64_t min_edges);
+/* Requests served through the triejoin so far. */
+int amj_server_triejoin_runs(amj_server_t* server, uint64_t* runs);
 /* Set the result budget before the first request (AMJ_ERR_STATE afterwards); the arena is
  * reserved again at that size then. */
 int amj_server_set_budget(amj_server_t* server, uint64_t hbm_budget_bytes);

@@ -180,6 +180,25 @@ def test_server_same_alone_and_in_mix(plain):
             assert st["batches"] < st["served"]  # requests were batched
 
 
+def test_server_triejoin_route_equals_hash_join_engines():
+    # set_triejoin: TRI / FOUR requests run alone through the leapfrog triejoin over a trie
+    # kept resident; the results (sorted tuples, count, checksum) equal the hash-join pattern
+    # engines' in a plain server, alone and in a mix, and the oracle's
+    with Server(0, max_batch=64, plain=True) as ref:
+        reqs, _chains, (src, dst) = _server_pool(ref)
+        want = [_key(ref.wait(_submit(ref, q), fetch=True)) for q in reqs]
+    with Server(0, max_batch=64) as srv:
+        srv.set_triejoin(1)
+        reqs, _chains, _g = _server_pool(srv)
+        alone = [_key(srv.wait(_submit(srv, q), fetch=True)) for q in reqs]
+        assert alone == want
+        tickets = [_submit(srv, q) for q in reqs * 2]
+        assert [_key(srv.wait(t, fetch=True)) for t in tickets] == want * 2
+        assert srv.tuning()["triejoin_alone"] == 6  # TRI and FOUR, three times each
+    assert want[-2][:2] == O.triangles(src, dst)
+    assert want[-1][:2] == O.four_cliques(src, dst)
+
+
 @pytest.mark.parametrize("plain", [False, True], ids=["batched", "plain"])
 def test_server_threads_and_budget_splits(plain):
     # a budget that holds only a few results at once forces batch splits (batched) or waits
