@@ -459,17 +459,28 @@ __device__ __forceinline__ void descend_clique_pairs(const LPlan& P, const LLv<A
       const uint32_t incl = wave_scan_add_dpp(len);
       const uint32_t excl = incl - len;
       const uint32_t T = lane63(incl);
-      for (uint32_t c = 0; c < T; c += (uint32_t)kWave) {
-        const uint32_t g = c + lane;
-        const int q = (int)parent_of_slots(excl, len, c);
-        const uint32_t k = g - (uint32_t)__shfl(excl, q, kWave);
-        const uint32_t qlo = (uint32_t)__shfl(clo, q, kWave);
-        bool ok[1] = {g < T};
-        int32_t a3[1] = {ok[0] ? ld_at<false>(r5.d1, qlo + k) : 0};
-        lds_has<1>(lst, nl, a3, ok);
-        b[2] = __shfl(a2, q, kWave);
-        b[3] = a3[0];
-        sink(ok[0], b);
+      for (uint32_t c = 0; c < T; c += (uint32_t)(K * kWave)) {
+        // K chunks of 64 candidates at once: their loads and searches are in flight together
+        int q[K];
+        bool ok[K];
+        int32_t a3[K];
+        sfor<0, K>([&](auto j) {
+          const uint32_t cj = c + (uint32_t)j * kWave;
+          const uint32_t g = cj + lane;
+          q[j] = (int)parent_of_slots(excl, len, cj < T ? cj : 0u);
+          const uint32_t k = g - (uint32_t)__shfl(excl, q[j], kWave);
+          const uint32_t qlo = (uint32_t)__shfl(clo, q[j], kWave);
+          ok[j] = g < T;
+          a3[j] = ok[j] ? ld_at<false>(r5.d1, qlo + k) : 0;
+        });
+        lds_has<K>(lst, nl, a3, ok);
+        sfor<0, K>([&](auto j) {
+          if (c + (uint32_t)j * kWave < T) {
+            b[2] = __shfl(a2, q[j], kWave);
+            b[3] = a3[j];
+            sink(ok[j], b);
+          }
+        });
       }
     }
   }
