@@ -157,39 +157,40 @@ struct amj_server {
       }
       return;
     }
-    if (tune_rows) {  // large chains alone, with their tuned engine
-      std::vector<Req> small;
-      for (auto& q : reqs) {
-        if (q.query == AMJ_QUERY_NORMAL && rels[q.ids[0]].rel->rows >= tune_rows) {
-          if (!c.deferred.empty()) c.deferred.push_back(q);
-          else run_one(c, q, true);
-        } else {
-          small.push_back(q);
-        }
-      }
-      reqs.swap(small);
-      if (reqs.empty()) return;
-      if (!c.deferred.empty()) {
-        c.deferred.insert(c.deferred.end(), reqs.begin(), reqs.end());
-        return;
+    // In submission order: runs of batchable requests go as one batch each, large chains alone
+    // with their tuned engine, large graph patterns alone through the triejoin. A request
+    // never starts before an earlier one: were a later result admitted first, an earlier
+    // request could wait for room that only the later result's release frees, while a client
+    // waits for the earlier result before it releases the later one.
+    std::vector<Req> run;
+    auto flush = [&] {
+      if (!run.empty()) run_batch_group(c, run);
+      run.clear();
+    };
+    for (auto& q : reqs) {
+      const uint64_t rows = rels[q.ids[0]].rel->rows;
+      const bool tuned = tune_rows && q.query == AMJ_QUERY_NORMAL && rows >= tune_rows;
+      const bool trie = trie_edges && q.query != AMJ_QUERY_NORMAL && rows >= trie_edges;
+      if (tuned || trie) {
+        flush();
+        if (!c.deferred.empty()) c.deferred.push_back(q);
+        else if (tuned) run_one(c, q, true);
+        else run_trie(c, q);
+      } else if (!c.deferred.empty()) {
+        c.deferred.push_back(q);
+      } else {
+        run.push_back(q);
       }
     }
-    if (trie_edges) {  // large graph patterns alone, through the triejoin
-      std::vector<Req> small;
-      for (auto& q : reqs) {
-        if (q.query != AMJ_QUERY_NORMAL && rels[q.ids[0]].rel->rows >= trie_edges) {
-          if (!c.deferred.empty()) c.deferred.push_back(q);
-          else run_trie(c, q);
-        } else {
-          small.push_back(q);
-        }
-      }
-      reqs.swap(small);
-      if (reqs.empty()) return;
-      if (!c.deferred.empty()) {
-        c.deferred.insert(c.deferred.end(), reqs.begin(), reqs.end());
-        return;
-      }
+    flush();
+  }
+
+  // Batchable requests (in order) as one batch, split in halves while their results do not fit.
+  void run_batch_group(Cycle& c, std::vector<Req> reqs) {
+    if (reqs.empty()) return;
+    if (!c.deferred.empty()) {
+      c.deferred.insert(c.deferred.end(), reqs.begin(), reqs.end());
+      return;
     }
     const double t0 = now_ms();
     for (auto& q : reqs) q.t_start = t0;
@@ -199,8 +200,8 @@ struct amj_server {
       ++c.splits;
       std::vector<Req> a(reqs.begin(), reqs.begin() + reqs.size() / 2);
       std::vector<Req> b(reqs.begin() + reqs.size() / 2, reqs.end());
-      run_group(c, a);
-      run_group(c, b);
+      run_batch_group(c, a);
+      run_batch_group(c, b);
     } else if (rc == AMJ_ERR_NOMEM && (need > budget || c.room >= budget)) {
       // larger than the budget, or no room even with nothing held: waiting cannot help
       c.out.push_back(failed(reqs[0], rc, "the result (" + std::to_string(need) +

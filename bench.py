@@ -310,7 +310,10 @@ def four_leg(G, scale):
     res = {}
     for name, fn in (("amhj", lambda: ctx.execute_pattern(rel, ht, "FOUR", materialize=False)),
                      ("alftj_dfs", lambda: ctx.lftj([t] * 6, [0, 1, 2, 3], "DFS",
-                                                    trie_attrs=four, materialize=False))):
+                                                    trie_attrs=four, materialize=False)),
+                     ("alftj_dfs_ws", lambda: ctx.lftj([t] * 6, [0, 1, 2, 3], "DFS",
+                                                       trie_attrs=four, materialize=False,
+                                                       ws=1))):
         t0 = time.perf_counter()
         r = fn()
         N = __import__("accmultiwayjoins_amd._native", fromlist=["lib"])
@@ -319,7 +322,8 @@ def four_leg(G, scale):
         res[name] = (r.count, r.checksum, el, r.probe_ms)
         log(f"[bench] four: {name} {r.count} 4-cliques in {el * 1e3:.0f} ms")
     (ca, sa, ea, pa), (cl, sl, el_, pl) = res["amhj"], res["alftj_dfs"]
-    if (ca, sa) != (cl, sl):
+    cw, sw, _ew, pws = res["alftj_dfs_ws"]
+    if not (ca, sa) == (cl, sl) == (cw, sw):
         raise SystemExit("bench: FOUR through AMHJ and ALFTJ disagree")
     t.close()
     ht.close()
@@ -328,9 +332,11 @@ def four_leg(G, scale):
     return {"count": ca, "checksum": "%#018x" % sa, "edges": len(src),
             "amhj": {"value": ca / (pa / 1e3), "unit": "tuples/s", "probe_ms": round(pa, 3)},
             "alftj_dfs": {"value": cl / (pl / 1e3), "unit": "tuples/s", "probe_ms": round(pl, 3)},
+            "alftj_dfs_ws": {"value": cw / (pws / 1e3), "unit": "tuples/s",
+                             "probe_ms": round(pws, 3)},
             "workload": f"FOUR (4-clique) on R-MAT scale {scale} edge factor 16, {len(src)} "
                         f"oriented edges, count + checksum (hash table / trie built untimed)",
-            "verified": "AMHJ and ALFTJ DFS agree (count + checksum)"}
+            "verified": "AMHJ, ALFTJ DFS and ALFTJ DFS with ws agree (count + checksum)"}
 
 
 def _serve(mode, sources, stream, gaps, budget, fit_budget=False, tune_rows=0, trie_edges=0):
@@ -673,8 +679,10 @@ def main():
     ap.add_argument("--no-ablation", action="store_true",
                     help="skip the thread-per-tuple baseline run on the first skew")
     ap.add_argument("--serve-queries", type=int, default=512)
-    ap.add_argument("--serve-rate", type=float, default=25.0,
-                    help="Poisson arrival rate of the configured stream (queries/s)")
+    ap.add_argument("--serve-rate", type=float, default=100.0,
+                    help="Poisson arrival rate of the configured stream (queries/s; above what "
+                         "either path serves, so that both legs measure throughput, while "
+                         "arrivals still overlap the batches on the GPU)")
     ap.add_argument("--serve-max-rows", type=int, default=50_000_000)
     ap.add_argument("--serve-small", action="store_true",
                     help="also serve the smaller stream over a few resident relations")
@@ -683,9 +691,9 @@ def main():
                     help="also run the out-of-core chain leg with this many rows per relation "
                          "(1500000000 for the 1.5 B-tuple case; off by default: ~1 min)")
     ap.add_argument("--ooc-budget-gb", type=float, default=48.0)
-    ap.add_argument("--four-scale", type=int, default=0,
-                    help="also run FOUR through AMHJ and ALFTJ on R-MAT of this scale (20 for "
-                         "the 4-clique case; off by default)")
+    ap.add_argument("--four-scale", type=int, default=20,
+                    help="also run FOUR through AMHJ and ALFTJ (with and without ws) on R-MAT of "
+                         "this scale (the 4-clique case; 0: off)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device to device three times (torch), to scale the "
                          "FETCH_SIZE / WRITE_SIZE counters of a profiled run, then exit")

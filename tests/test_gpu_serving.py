@@ -199,11 +199,17 @@ def test_server_triejoin_route_equals_hash_join_engines():
     assert want[-1][:2] == O.four_cliques(src, dst)
 
 
-@pytest.mark.parametrize("plain", [False, True], ids=["batched", "plain"])
-def test_server_threads_and_budget_splits(plain):
+@pytest.mark.parametrize("mode", ["batched", "plain", "routes"])
+def test_server_threads_and_budget_splits(mode):
     # a budget that holds only a few results at once forces batch splits (batched) or waits
-    # for releases (plain); every request still gets its result
-    with Server(0, hbm_budget_bytes=16 << 20, max_batch=64, plain=plain) as srv:
+    # for releases (plain); every request still gets its result. "routes": larger chains run
+    # alone with tuned options and the graph patterns through the triejoin, between batches;
+    # requests still start in submission order, so clients that wait for their tickets in
+    # order never wait for room held by a later result
+    with Server(0, hbm_budget_bytes=16 << 20, max_batch=64, plain=mode == "plain") as srv:
+        if mode == "routes":
+            srv.set_autotune(10000)
+            srv.set_triejoin(1)
         reqs, _chains, _g = _server_pool(srv)
         alone = [_key(srv.wait(_submit(srv, q), fetch=True)) for q in reqs]
         results = {}
