@@ -286,7 +286,9 @@ hipError_t launch_tpt(const Plan& plan, bool write, bool hash, uint64_t* counts,
 hipError_t launch_fib_weights(const Plan& plan, int level, uint64_t* w, RunRef* next,
                               hipStream_t s);
 hipError_t launch_fib_probe_weights(const Plan& plan, uint64_t* w, RunRef* next, hipStream_t s);
-// Output-window writer: every wave materialises outputs [w*win, (w+1)*win) exactly.
+// Output-window writer: every wave materialises outputs [w*win, (w+1)*win) exactly; win is a
+// multiple of kWindowAlign rows (the writer stages and stores aligned blocks of that many rows).
+constexpr uint64_t kWindowAlign = 256;
 uint64_t window_blocks(uint64_t total, uint64_t window);
 bool window_plan_supported(const Plan& plan);
 hipError_t launch_records_to_suffix(RunRef* rec, uint64_t n, const uint64_t* pw, hipStream_t s);

@@ -161,9 +161,10 @@ int run_fib(amj_ctx_t* ctx, const Plan& plan, const amj_opts_t* o, amj_result_t*
   uint64_t sstride[kMaxLevels];
   sbase[L - 1] = P.tab[L - 1].vals;
   sstride[L - 1] = P.tab[L - 1].n;
-  // windows are whole 64-row blocks (the writer stages and flushes aligned 64-row blocks)
-  const uint64_t window =
-      o->out_chunk_rows ? (o->out_chunk_rows + 63) / 64 * 64 : kDefaultWindow;
+  // windows are whole staging blocks of the writer
+  const uint64_t window = o->out_chunk_rows
+                              ? (o->out_chunk_rows + kWindowAlign - 1) / kWindowAlign * kWindowAlign
+                              : kDefaultWindow;
   for (int t = L - 2; t >= k; --t) {
     void* buf = nullptr;
     AMJ_TRY(scratch(ctx, SLOT_SFX0 + t, std::max<uint64_t>(srows[t], 1) * ncols[t] * 4, &buf));

@@ -87,7 +87,7 @@ typedef struct {
                                    materialize = 0) */
   uint64_t hbm_budget_bytes;    /* 0: use what the device has free */
   uint64_t ooc_slice_rows;      /* 0: derive from the budget */
-  uint64_t out_chunk_rows;      /* fib: rows per output window (0: 4,096) */
+  uint64_t out_chunk_rows;      /* fib: rows per output window, rounded up to a multiple of 256 (0: 4,096) */
   uint64_t dro_capacity_rows;   /* dro: initial output allocation (0: sampled estimate) */
   uint32_t ws_split_threshold;  /* ws: split runs longer than this (0: default) */
   int fib_suffix;               /* fib: 0 size-based suffix tables, 1 none, 2 deepest that fits */
