@@ -77,6 +77,60 @@ def cpu_baseline(rels, sample_rows, threads):
     return cnt, (pms + qms) / 1e3, wall
 
 
+def config1_leg(ctx, G, threads, reps=5):
+    """The CPU-runnable configuration at full size: the 3-way chain R1(a,b) x R2(b,c) x R3(c,d),
+    1 M tuples of two int32 columns per relation, join keys uniform over [0, 1 M), seed 42.
+    The oracle's MHJ-CPU restatement runs on the host cores (partition + probe, count +
+    checksum), and plain AMHJ on the GPU (ooc=0 ws=0 dro=0 fib=0: count pass, scan, write pass)
+    over relations already in HBM: hash builds + probe + materialised output, best of `reps`.
+    Both must equal the MHJ-CPU count of the reference's own build recorded in
+    tests/golden/mhj_cpu_counts.json."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_ffi as O
+    N = __import__("accmultiwayjoins_amd._native", fromlist=["lib"])
+    rels = G.chain_relations([1_000_000] * 3, 1_000_000, seed=42)
+    with open(os.path.join(ROOT, "tests", "golden", "mhj_cpu_counts.json")) as f:
+        golden = next(d["count"] for d in json.load(f)["datasets"]
+                      if d["name"] == "c1_3x1M_uniform_seed42")
+    t0 = time.perf_counter()
+    cnt, cs, _, pms, qms = O.mhj(rels, threads=threads)
+    cpu_wall = time.perf_counter() - t0
+    R = [ctx.relation(c, ids) for c, ids in rels]
+    best = None
+    try:
+        for _ in range(reps):
+            N.lib().amj_sync(ctx.h)
+            t = time.perf_counter()
+            H = [ctx.hash_table(r) for r in R[1:]]
+            r = ctx.execute(R[0], H, materialize=True, checksum=True, ws=0, dro=0, fib=0,
+                            ooc=0)
+            N.lib().amj_sync(ctx.h)
+            el = time.perf_counter() - t
+            g = (r.count, r.checksum, r.rows_written, r.path)
+            ctx.free(r)
+            for h in H:
+                h.close()
+            best = el if best is None else min(best, el)
+    finally:
+        for x in R:
+            x.close()
+    if not (g[0] == cnt == golden and g[1] == cs and g[2] == cnt):
+        raise SystemExit("bench: config 1 differs between the GPU, the oracle and the "
+                         "reference's recorded count")
+    log(f"[bench] config 1: {cnt} results; CPU {(pms + qms):.1f} ms ({threads} threads), "
+        f"GPU plain {best * 1e3:.2f} ms")
+    return {"count": cnt, "checksum": "%#018x" % cs, "reference_count": golden,
+            "cpu": {"value": cnt / ((pms + qms) / 1e3), "unit": "tuples/s", "cores": threads,
+                    "kind": "port", "ms": round(pms + qms, 3),
+                    "wall_ms": round(cpu_wall * 1e3, 3)},
+            "gpu_plain": {"value": cnt / best, "unit": "tuples/s", "ms": round(best * 1e3, 3),
+                          "path": g[3],
+                          "timed": "2 hash builds + count pass + scan + write pass, "
+                                   "materialised, relations resident"},
+            "workload": "3-way chain, 1 M tuples per relation, keys uniform over [0, 1 M), "
+                        "seed 42 (full size)"}
+
+
 def host_cpu():
     """(model name, logical CPUs, SMT active) of the host."""
     model = "unknown"
@@ -826,6 +880,7 @@ def main():
                "host": {"cpu_model": model, "logical_cpus": ncpu, "smt_active": smt}}
         log(f"[bench] cpu baseline: {cnt} tuples in {secs:.2f} s ({wall:.2f} s wall); "
             f"1 thread: {cnt1} in {secs1:.2f} s")
+        cpu["config1"] = config1_leg(ctx, G, args.cpu_threads)
 
     out = {
         "metric": "join output tuples/s (4-way chain)",
