@@ -7,14 +7,17 @@
 // (:340-454, with a host OpenMP load-balance search). It returns an accumulator that nothing
 // updates, so the count is never reported (:496-499).
 //
-// Here every level is materialised explicitly: level 0 keeps the surviving roots of the
-// smallest level-0 list; at level i every frontier entry picks its shortest candidate list, a
-// scan of those lengths flattens all candidates, and one thread per candidate finds its entry by
-// binary search over the scan, checks the candidate against the other lists and either counts
-// it (last attribute) or writes the extended entry, compacted by a scan of the survivor flags.
-// Candidates are processed in batches sized to the memory budget. The count and checksum are
+// Here every level but the last is materialised explicitly: level 0 keeps the surviving roots
+// of the smallest level-0 list; at level i every frontier entry picks its shortest candidate
+// list, a scan of those lengths flattens all candidates, and one thread per candidate finds its
+// entry by binary search over the scan, checks the candidate against the other lists and writes
+// the extended entry, compacted by a scan of the survivor flags (candidates in batches sized to
+// the memory budget). The last attribute is counted from the materialised frontier 64 entries
+// per wave, by the DFS engine's last level (lftj_dfs.h; for a triangle or a 4-clique its
+// cooperative form, the other lists staged in LDS): one thread per last-level candidate had to
+// find its entry by a binary search over the whole frontier first. The count and checksum are
 // returned; any query the DFS engine runs also runs here, and the two must agree.
-#include "lftj_common.h"
+#include "lftj_dfs.h"
 #include "runtime.h"
 
 namespace amj {
@@ -98,13 +101,13 @@ __global__ __launch_bounds__(kBlock) void bfs_sizes_kernel(const LPlan P, int i,
   }
 }
 
-// Candidates [g0, g1) of attribute i: entry by binary search over the scanned lengths.
-template <int A, bool LAST>
+// Candidates [g0, g1) of attribute i (not the last): entry by binary search over the scanned
+// lengths; the extended entry and its survivor flag are written at g - g0.
+template <int A>
 __global__ __launch_bounds__(kBlock) void bfs_expand_kernel(
     const LPlan P, int i, const BfsEntry<A>* __restrict__ f, uint64_t n,
     const uint64_t* __restrict__ off, uint64_t g0, uint64_t g1, BfsEntry<A>* __restrict__ next,
-    uint32_t* __restrict__ keep, uint64_t* __restrict__ block_sums) {
-  uint64_t cnt = 0, csum = 0;
+    uint32_t* __restrict__ keep) {
   for (uint64_t g = g0 + (uint64_t)blockIdx.x * kBlock + threadIdx.x; g < g1;
        g += (uint64_t)gridDim.x * kBlock) {
     uint64_t lo = 0, hi = n;  // last e with off[e] <= g
@@ -121,24 +124,64 @@ __global__ __launch_bounds__(kBlock) void bfs_expand_kernel(
     sfor<0, A>([&](auto a) {  // l.b[i] = v without a dynamically indexed (scratch) array
       if ((int)a == i) l.b[a] = v;
     });
-    if (LAST) {
-      if (ok) {
-        ++cnt;
-        int32_t w[kMaxAttrs];
-        sfor<0, kMaxAttrs>([&](auto a) { w[a] = (a < A) ? l.b[(a < A) ? (int)a : 0] : 0; });
-        csum += hash_tuple(w, A);
-      }
+    BfsEntry<A> e;
+    from_lv(l, e);
+    next[g - g0] = e;
+    keep[g - g0] = ok ? 1u : 0u;
+  }
+}
+
+// The last attribute over a materialised frontier of n entries, 64 per wave: count and
+// checksum (per block: block_sums[2b], block_sums[2b + 1]).
+template <int A, uint64_t SH>
+__global__ __launch_bounds__(kBlock) void bfs_last_kernel(const LPlan P,
+                                                          const BfsEntry<A>* __restrict__ f,
+                                                          uint64_t n,
+                                                          uint64_t* __restrict__ block_sums) {
+  __shared__ int32_t coop_mem[SH != 0 ? kWavesPerBlock * stage_ints<SH>() : 1];
+  lds_i32* stage =
+      SH != 0 ? (lds_i32*)&coop_mem[(threadIdx.x / kWave) * stage_ints<SH>()] : nullptr;
+  const uint64_t w = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+  CountHashSink<true> sink;
+  if (w * kWave < n) {  // wave-uniform
+    const uint64_t e = w * kWave + (uint64_t)lane_id();
+    const bool valid = e < n;
+    LLv<A> l;
+    if (valid) {
+      to_lv(f[e], l);
     } else {
-      BfsEntry<A> e;
-      from_lv(l, e);
-      next[g - g0] = e;
-      keep[g - g0] = ok ? 1u : 0u;
+      sfor<0, A>([&](auto a) { l.b[a] = 0; });
+      sfor<0, kR>([&](auto q) { l.lo[q] = 0u; l.hi[q] = 0u; });
+    }
+    choose_driver<A, SH>(P, A - 1, valid, l);
+    finish(l);
+    if (l.T) descend<A, A - 1, SH>(P, l, sink, stage);
+  }
+  block_sum_store(lane_id() == 0 ? sink.count : 0, &block_sums[2 * blockIdx.x]);
+  block_sum_store(sink.csum, &block_sums[2 * blockIdx.x + 1]);
+}
+
+template <int A>
+static hipError_t launch_bfs_last(const LPlan& P, const BfsEntry<A>* f, uint64_t n,
+                                  uint64_t* block_sums, unsigned blocks, hipStream_t s) {
+  const uint64_t sh = lftj_static_shape(P);
+  if constexpr (A == 3) {
+    if (sh == kShapeTri) {
+      hipLaunchKernelGGL((bfs_last_kernel<3, kShapeTri>), dim3(blocks), dim3(kBlock), 0, s, P,
+                         f, n, block_sums);
+      return hipGetLastError();
     }
   }
-  if (LAST) {
-    block_sum_store(cnt, &block_sums[2 * blockIdx.x]);
-    block_sum_store(csum, &block_sums[2 * blockIdx.x + 1]);
+  if constexpr (A == 4) {
+    if (sh == kShapeFour) {
+      hipLaunchKernelGGL((bfs_last_kernel<4, kShapeFour>), dim3(blocks), dim3(kBlock), 0, s, P,
+                         f, n, block_sums);
+      return hipGetLastError();
+    }
   }
+  hipLaunchKernelGGL((bfs_last_kernel<A, 0>), dim3(blocks), dim3(kBlock), 0, s, P, f, n,
+                     block_sums);
+  return hipGetLastError();
 }
 
 static unsigned grid_n(uint64_t n) {
@@ -202,6 +245,22 @@ static int bfs_run(amj_ctx_t* ctx, const LPlan& P, uint64_t budget, uint64_t* co
     n = m;
     for (int i = 1; i < A && n > 0; ++i) {
       const bool last = i == A - 1;
+      if (last) {  // counted from the frontier, 64 entries per wave
+        const uint64_t waves = (n + kWave - 1) / kWave;
+        const uint64_t blocks = (waves + kWavesPerBlock - 1) / kWavesPerBlock;
+        void* bs = nullptr;
+        if ((rc = scratch(ctx, SLOT_BLOCK_SUMS, blocks * 16, &bs)) != AMJ_OK) break;
+        AMJ_HIP_TRY(ctx, launch_bfs_last<A>(P, (const E*)cur.p, n, (uint64_t*)bs,
+                                            (unsigned)blocks, s));
+        std::vector<uint64_t> h((size_t)blocks * 2);
+        AMJ_HIP_TRY(ctx, hipMemcpyAsync(h.data(), bs, h.size() * 8, hipMemcpyDeviceToHost, s));
+        AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
+        for (uint64_t b = 0; b < blocks; ++b) {
+          total += h[2 * b];
+          csum += h[2 * b + 1];
+        }
+        break;
+      }
       if ((rc = grow(ctx, len, (n + 1) * 8, 0)) != AMJ_OK) break;
       if ((rc = scratch(ctx, SLOT_SCAN_TEMP, scan_temp_elems(n + 1) * 8, &stmp)) != AMJ_OK) break;
       hipLaunchKernelGGL(bfs_sizes_kernel<A>, dim3(grid_n(n + 1)), dim3(kBlock), 0, s, P, i,
@@ -220,49 +279,32 @@ static int bfs_run(amj_ctx_t* ctx, const LPlan& P, uint64_t budget, uint64_t* co
         const uint64_t g1 = std::min<uint64_t>(C, g0 + batch);
         const uint64_t nb = g1 - g0;
         const unsigned grid = grid_n(nb);
-        if (last) {
-          void* bs = nullptr;
-          if ((rc = scratch(ctx, SLOT_BLOCK_SUMS, (uint64_t)grid * 16, &bs)) != AMJ_OK) break;
-          hipLaunchKernelGGL((bfs_expand_kernel<A, true>), dim3(grid), dim3(kBlock), 0, s, P, i,
-                             (const E*)cur.p, n, (const uint64_t*)len.p, g0, g1, (E*)nullptr,
-                             (uint32_t*)nullptr, (uint64_t*)bs);
-          std::vector<uint64_t> h((size_t)grid * 2);
-          AMJ_HIP_TRY(ctx, hipMemcpyAsync(h.data(), bs, h.size() * 8, hipMemcpyDeviceToHost, s));
-          AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
-          for (unsigned b = 0; b < grid; ++b) {
-            total += h[2 * b];
-            csum += h[2 * b + 1];
-          }
-        } else {
-          if ((rc = grow(ctx, tmp, nb * sizeof(E), 0)) != AMJ_OK) break;
-          if ((rc = grow(ctx, keep, (nb + 1) * 4, 0)) != AMJ_OK) break;
-          if ((rc = grow(ctx, pos, (nb + 1) * 8, 0)) != AMJ_OK) break;
-          if ((rc = scratch(ctx, SLOT_SCAN_TEMP, scan_temp_elems(nb + 1) * 8, &stmp)) != AMJ_OK)
-            break;
-          hipLaunchKernelGGL((bfs_expand_kernel<A, false>), dim3(grid), dim3(kBlock), 0, s, P, i,
-                             (const E*)cur.p, n, (const uint64_t*)len.p, g0, g1, (E*)tmp.p,
-                             (uint32_t*)keep.p, (uint64_t*)nullptr);
-          hipLaunchKernelGGL(bfs_u32_to_u64_kernel, dim3(grid_n(nb + 1)), dim3(kBlock), 0, s,
-                             (const uint32_t*)keep.p, nb, (uint64_t*)pos.p);
-          AMJ_HIP_TRY(ctx, exclusive_scan_u64((uint64_t*)pos.p, (uint64_t*)pos.p, nb + 1,
-                                              (uint64_t*)stmp, s));
-          uint64_t m2 = 0;
-          AMJ_HIP_TRY(ctx, hipMemcpyAsync(&m2, (uint64_t*)pos.p + nb, 8, hipMemcpyDeviceToHost, s));
-          AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
-          if ((rc = grow(ctx, nxt, std::max<uint64_t>(next_n + m2, 1) * sizeof(E),
-                         next_n * sizeof(E))) != AMJ_OK)
-            break;
-          hipLaunchKernelGGL(bfs_compact_kernel<E>, dim3(grid), dim3(kBlock), 0, s,
-                             (const E*)tmp.p, (const uint32_t*)keep.p, (const uint64_t*)pos.p, nb,
-                             (E*)nxt.p + next_n);
-          next_n += m2;
-        }
+        if ((rc = grow(ctx, tmp, nb * sizeof(E), 0)) != AMJ_OK) break;
+        if ((rc = grow(ctx, keep, (nb + 1) * 4, 0)) != AMJ_OK) break;
+        if ((rc = grow(ctx, pos, (nb + 1) * 8, 0)) != AMJ_OK) break;
+        if ((rc = scratch(ctx, SLOT_SCAN_TEMP, scan_temp_elems(nb + 1) * 8, &stmp)) != AMJ_OK)
+          break;
+        hipLaunchKernelGGL((bfs_expand_kernel<A>), dim3(grid), dim3(kBlock), 0, s, P, i,
+                           (const E*)cur.p, n, (const uint64_t*)len.p, g0, g1, (E*)tmp.p,
+                           (uint32_t*)keep.p);
+        hipLaunchKernelGGL(bfs_u32_to_u64_kernel, dim3(grid_n(nb + 1)), dim3(kBlock), 0, s,
+                           (const uint32_t*)keep.p, nb, (uint64_t*)pos.p);
+        AMJ_HIP_TRY(ctx, exclusive_scan_u64((uint64_t*)pos.p, (uint64_t*)pos.p, nb + 1,
+                                            (uint64_t*)stmp, s));
+        uint64_t m2 = 0;
+        AMJ_HIP_TRY(ctx, hipMemcpyAsync(&m2, (uint64_t*)pos.p + nb, 8, hipMemcpyDeviceToHost, s));
+        AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
+        if ((rc = grow(ctx, nxt, std::max<uint64_t>(next_n + m2, 1) * sizeof(E),
+                       next_n * sizeof(E))) != AMJ_OK)
+          break;
+        hipLaunchKernelGGL(bfs_compact_kernel<E>, dim3(grid), dim3(kBlock), 0, s,
+                           (const E*)tmp.p, (const uint32_t*)keep.p, (const uint64_t*)pos.p, nb,
+                           (E*)nxt.p + next_n);
+        next_n += m2;
       }
       if (rc != AMJ_OK) break;
-      if (!last) {
-        std::swap(cur, nxt);
-        n = next_n;
-      }
+      std::swap(cur, nxt);
+      n = next_n;
     }
   } while (false);
   if (rc == AMJ_OK) {
