@@ -231,24 +231,30 @@ __device__ __forceinline__ uint64_t readlane_u64(uint64_t v, int lane) {
 }
 
 // Are v[j] in the ascending run st[0, n) of LDS (n the same on every lane)? The K searches
-// advance together, so K reads are in flight at every step; ok[j] is cleared for a miss.
+// advance together, so K reads are in flight at every step; ok[j] is cleared for a miss. The
+// position is kept as an LDS pointer, so a step is one add, one read, one compare and one
+// select per search.
 template <int K>
 __device__ __forceinline__ void lds_has(const lds_i32* st, uint32_t n, const int32_t (&v)[K],
                                         bool (&ok)[K]) {
-  uint32_t b[K];
-  sfor<0, K>([&](auto j) { b[j] = 0u; });
   if (n == 0) {
     sfor<0, K>([&](auto j) { ok[j] = false; });
     return;
   }
+  const lds_i32* b[K];
+  sfor<0, K>([&](auto j) { b[j] = st; });
   while (n > 1) {
     const uint32_t half = n >> 1;
+    const lds_i32* c[K];
     int32_t x[K];
-    sfor<0, K>([&](auto j) { x[j] = st[b[j] + half]; });
-    sfor<0, K>([&](auto j) { b[j] = x[j] <= v[j] ? b[j] + half : b[j]; });
+    sfor<0, K>([&](auto j) {
+      c[j] = b[j] + half;
+      x[j] = *c[j];
+    });
+    sfor<0, K>([&](auto j) { b[j] = x[j] <= v[j] ? c[j] : b[j]; });
     n -= half;
   }
-  sfor<0, K>([&](auto j) { ok[j] = ok[j] && st[b[j]] == v[j]; });
+  sfor<0, K>([&](auto j) { ok[j] = ok[j] && *b[j] == v[j]; });
 }
 
 // Chunks of 64 candidates a wave checks at once in the cooperative last level.
