@@ -357,8 +357,9 @@ class Context:
         return self._finish(res, fetch)
 
     def autotune(self, probe: Relation, tables: Sequence[HashTable], **opts) -> dict:
-        """Engine options chosen by timing candidates on a sample of the probe relation
-        (amj_autotune); pass them back to execute()."""
+        """Engine options chosen by timing candidates on two samples of the probe relation
+        (amj_autotune); `estimated_ms` is the pick's device time extrapolated to the whole
+        relation. Pass the options back to execute()."""
         hts = (C.c_void_p * len(tables))(*[t.h.value for t in tables])
         o = _opts(**opts)
         ms = C.c_double()
@@ -366,8 +367,9 @@ class Context:
                                          C.byref(ms)))
         return {"ws": o.ws, "dro": o.dro, "fib": o.fib, "out_chunk_rows": o.out_chunk_rows,
                 "fib_suffix": o.fib_suffix, "ws_split_threshold": o.ws_split_threshold,
+                "ws_claim_units": o.ws_claim_units,
                 "materialize": bool(o.materialize), "checksum": bool(o.checksum),
-                "sample_ms": ms.value}
+                "estimated_ms": ms.value}
 
     def execute_pattern(self, edges: Relation, table: HashTable, query: str = "TRI",
                         fetch: bool = False, **opts) -> JoinResult:

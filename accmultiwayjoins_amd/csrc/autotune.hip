@@ -4,9 +4,10 @@
 // Knobs (the reference fixes all of them at compile time, MHJ-GPU/conf.h): the engine (count /
 // write unit engine, persistent ws engine, dro), the ws task size (how many waves share a heavy
 // key's run) and claim size (units a wave takes at once), and for frequent-item chains the
-// output-window size (rows per wave step) and the suffix-table mode. Candidates run on an evenly
-// strided sample of at most 4 M probe rows with the caller's materialise setting; the fastest
-// (device time, best of two runs) wins and is remembered per query shape. The hash tables'
+// output-window size (rows per wave step) and the suffix-table mode. Candidates run with the
+// caller's materialise setting on two evenly strided samples of the probe relation (at most 2 M
+// and 4 M rows); the device time (best of two runs on each) extrapolated to the whole relation
+// decides, and the pick is remembered per query shape. The hash tables'
 // bucket count stays the reference's rule (B = floorPow2(rows), QueryProcessor.cuh:86), which
 // the tables were built with before the query.
 #include <cstring>
@@ -66,22 +67,34 @@ int amj_autotune(amj_ctx_t* ctx, const amj_relation_t* probe, amj_hash_table_t* 
     if (tuned_ms) *tuned_ms = hit->second.ms;
     return AMJ_OK;
   }
-  // evenly strided sample of the probe relation
+  // Two evenly strided samples of the probe relation, of at most 2 M and 4 M rows (the
+  // relation itself when it holds at most 4 M rows). A candidate's device time is taken as
+  // affine in the probe rows: costs that scale with the build tables (the suffix tables of the
+  // frequent-item path) plus a cost per probe row. Timed on both samples, it is extrapolated
+  // to the whole relation, so a fixed cost counts once, as it does in the real run.
+  constexpr uint64_t kLarge = 4ull << 20;
   const uint64_t n = probe->rows;
-  const uint64_t stride = n > (4ull << 20) ? (n + (4ull << 20) - 1) / (4ull << 20) : 1;
-  const uint64_t ns = (n + stride - 1) / stride;
-  amj_relation sample = *probe;
-  sample.owned = false;
-  void* buf = nullptr;
-  if (stride > 1) {
-    AMJ_TRY(scratch(ctx, SLOT_MISC2, ns * 4 * probe->num_cols, &buf));
+  const bool two = n > kLarge;
+  const uint64_t s_large = two ? (n + kLarge - 1) / kLarge : 1, s_small = 2 * s_large;
+  const uint64_t n_large = (n + s_large - 1) / s_large, n_small = (n + s_small - 1) / s_small;
+  amj_relation large = *probe, small = *probe;
+  large.owned = small.owned = false;
+  if (two) {
+    void* buf = nullptr;
+    AMJ_TRY(scratch(ctx, SLOT_TUNE_SAMPLE, (n_large + n_small) * 4 * probe->num_cols, &buf));
+    int32_t* dst = (int32_t*)buf;
     for (uint32_t c = 0; c < probe->num_cols; ++c) {
-      int32_t* dst = (int32_t*)buf + c * ns;
       hipLaunchKernelGGL(gather_stride_kernel, dim3(1024), dim3(256), 0, ctx->stream,
-                         probe->cols[c], ns, stride, dst);
-      sample.cols[c] = dst;
+                         probe->cols[c], n_large, s_large, dst);
+      large.cols[c] = dst;
+      dst += n_large;
+      hipLaunchKernelGGL(gather_stride_kernel, dim3(1024), dim3(256), 0, ctx->stream,
+                         probe->cols[c], n_small, s_small, dst);
+      small.cols[c] = dst;
+      dst += n_small;
     }
-    sample.rows = ns;
+    large.rows = n_large;
+    small.rows = n_small;
     AMJ_HIP_TRY(ctx, hipGetLastError());
   }
   // candidates: the engine, the persistent engine's task size (waves per heavy key) and claim
@@ -109,19 +122,27 @@ int amj_autotune(amj_ctx_t* ctx, const amj_relation_t* probe, amj_hash_table_t* 
   if (fib_eligible(full) && window_plan_supported(full))
     for (uint64_t w : {1024ull, 4096ull, 16384ull})
       for (int sfx : {0, 1}) add(1, 1, 1, w, sfx, 0, 0);
-  double best = 1e300;
-  amj_opts_t pick = base;
-  for (const auto& c : cands) {
+  // best of two runs (the first may also allocate scratch)
+  auto time_on = [&](const amj_relation& rel, const amj_opts_t& c) {
     double t = 1e300;
     for (int rep = 0; rep < 2; ++rep) {
       amj_result_t r;
-      const int rc = amj_execute(ctx, &sample, hts, n_hts, &c, &r);
-      if (rc != AMJ_OK) {
-        t = 1e300;
-        break;
-      }
+      if (amj_execute(ctx, &rel, hts, n_hts, &c, &r) != AMJ_OK) return 1e300;
       t = std::min(t, r.probe_ms);
       amj_result_free(ctx, &r);
+    }
+    return t;
+  };
+  double best = 1e300;
+  amj_opts_t pick = base;
+  for (const auto& c : cands) {
+    const double t_large = time_on(large, c);
+    if (t_large >= 1e300) continue;
+    double t = t_large;
+    if (two) {
+      const double t_small = time_on(small, c);
+      if (t_small >= 1e300) continue;
+      t += std::max(0.0, t_large - t_small) * (double)(n - n_large) / (double)(n_large - n_small);
     }
     if (t < best) {
       best = t;

@@ -95,6 +95,7 @@ enum ScratchSlot {
   SLOT_DYN_CTL,    // persistent probe: control block, task records, task flags
   SLOT_DYN_TASKS,
   SLOT_DYN_READY,
+  SLOT_TUNE_SAMPLE,  // the autotuner's probe samples (the engines it runs use the others)
 };
 
 int dmalloc(amj_ctx_t* ctx, void** p, size_t bytes);

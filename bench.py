@@ -258,6 +258,41 @@ def triangle_leg(ctx, G, scale, steps, warmup):
                          "verified_against": "ALFTJ DFS (count + checksum)"}}
 
 
+def autotune_leg(ctx, R, skew, default):
+    """The autotuner's pick (amj_autotune: engine, task and claim sizes, direct output, output
+    window, suffix tables, timed on two strided samples of the probe relation and extrapolated
+    to all of it) for the chain at one skew, then the whole join run with the picked options
+    beside the default (untimed for `value`); both must give the default run's count and
+    checksum."""
+    H = [ctx.hash_table(r) for r in R[1:]]
+    try:
+        t0 = time.perf_counter()
+        pick = ctx.autotune(R[0], H, materialize=True, checksum=True)
+        tune_s = time.perf_counter() - t0
+        opts = {k: pick[k] for k in ("ws", "dro", "fib", "out_chunk_rows", "fib_suffix",
+                                     "ws_split_threshold", "ws_claim_units")}
+        r = ctx.execute(R[0], H, materialize=True, checksum=False, **opts)  # as timed steps
+        ms, path = r.probe_ms, r.path
+        ctx.free(r)
+        r = ctx.execute(R[0], H, materialize=True, checksum=True, **opts)
+        got = (r.count, "%#018x" % r.checksum, ms, path)
+        ctx.free(r)
+    finally:
+        for h in H:
+            h.close()
+    if (got[0], got[1]) != (default["count"], default["checksum"]):
+        raise SystemExit(f"bench: the autotuned options gave {got[0]}/{got[1]}, the default "
+                         f"{default['count']}/{default['checksum']}")
+    log(f"[bench] autotune (s={skew}): {opts}, estimated {pick['estimated_ms']:.1f} ms; "
+        f"whole join {got[2]:.1f} ms ({got[3]}) vs default {default['probe_ms']:.1f} ms")
+    return {"skew": skew, "choice": opts, "path": got[3],
+            "estimated_probe_ms": round(pick["estimated_ms"], 3), "tuning_s": round(tune_s, 3),
+            "tuned_probe_ms": round(got[2], 3), "default_probe_ms": default["probe_ms"],
+            "workload": "the chain of this skew; candidates timed on evenly strided 2 M- and "
+                        "4 M-row samples of R1 (materialised), extrapolated to all of R1",
+            "verified": "count + checksum equal to the default run"}
+
+
 def ablation_leg(ctx, resident, s, checked, wave_probe_ms, G):
     """The comparison engines on the same resident chain at skew s, materialising, against the
     wave engine's probe time from the timed steps (counts and checksums must equal the verified
@@ -849,6 +884,8 @@ def main():
                          "verified_against": checked[s][3]}
                 for (s, cnt, cs, bms, pms, kms, path, _x) in last}
 
+    tuned = autotune_leg(ctx, resident[skews[0]], skews[0], per_skew[str(skews[0])])
+
     ablation = None
     if not args.no_ablation:
         ablation = ablation_leg(ctx, resident, skews[0], checked[skews[0]],
@@ -906,6 +943,7 @@ def main():
                      "traffic_source": os.path.relpath(TRAFFIC_PROFILE, ROOT)},
         "cpu_baseline": cpu,
         "ablation": ablation,
+        "autotune": tuned,
         "triangle": tri,
         "serving": None,
     }

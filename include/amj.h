@@ -239,12 +239,14 @@ int amj_result_copy_col(amj_ctx_t* ctx, const amj_result_t* res, uint32_t col, u
 
 /* ---- autotuner ---------------------------------------------------------------------------- */
 /* No counterpart in the reference (its schedule is fixed at compile time, MHJ-GPU/conf.h).
- * Times candidate engine options on an evenly strided sample (at most 4 M rows) of the probe
- * relation with the caller's materialise setting: the unit engine, the persistent engine with
- * small and large tasks, direct output, and for frequent-item chains the output-window size
- * (16 Ki / 64 Ki / 256 Ki rows) with and without suffix tables. The fastest is written into
- * *opts (materialize and checksum kept) and remembered in the context for the same query
- * shape; tuned_ms is its device time on the sample. */
+ * Times candidate engine options with the caller's materialise setting on two evenly strided
+ * samples of the probe relation (at most 2 M and 4 M rows; the relation itself up to 4 M rows):
+ * the unit engine, the persistent engine with small and large tasks and claims, direct output,
+ * and for frequent-item chains the output-window size (1 Ki / 4 Ki / 16 Ki rows) with and
+ * without suffix tables. Each candidate's device time is extrapolated from the two samples to
+ * the whole relation (affine in the probe rows: fixed costs such as suffix tables count once).
+ * The fastest is written into *opts (materialize and checksum kept) and remembered in the
+ * context for the same query shape; tuned_ms is its extrapolated device time. */
 int amj_autotune(amj_ctx_t* ctx, const amj_relation_t* probe, amj_hash_table_t* const* hts,
                  uint32_t n_hts, amj_opts_t* opts, double* tuned_ms);
 void amj_autotune_forget(amj_ctx_t* ctx);

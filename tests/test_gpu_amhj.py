@@ -318,9 +318,29 @@ def test_autotune_choice_gives_the_same_result(ctx):
     H = [ctx.hash_table(r) for r in R[1:]]
     want = ctx.execute(R[0], H, materialize=False)
     tuned = ctx.autotune(R[0], H, materialize=True)
-    opts = {k: v for k, v in tuned.items() if k != "sample_ms"}
+    opts = {k: v for k, v in tuned.items() if k != "estimated_ms"}
     r = ctx.execute(R[0], H, **opts)
     assert (r.count, r.checksum) == (want.count, want.checksum)
     ctx.free(r)
     again = ctx.autotune(R[0], H, materialize=True)  # remembered for this query shape
     assert again == tuned
+
+
+def test_autotune_two_samples_of_a_large_probe(ctx):
+    # above 4 M probe rows the candidates run on two strided samples (their engines' own
+    # scratch, such as direct output's chunk tails, must leave the samples intact) and the
+    # pick is the best extrapolated time; the picked options give the plain engine's result
+    rels = G.chain_relations([9 << 20, 1 << 18, 1 << 18, 1 << 18], 1 << 19, seed=8, zipf_s=0.5,
+                             payload_range=1 << 31)
+    R = [ctx.relation(c, ids) for c, ids in rels]
+    H = [ctx.hash_table(r) for r in R[1:]]
+    want = ctx.execute(R[0], H, materialize=False)
+    assert want.count > 0 and (want.count, want.checksum) == tuple(O.mhj(rels)[:2])
+    for mat in (True, False):
+        tuned = ctx.autotune(R[0], H, materialize=mat)
+        assert 0 < tuned["estimated_ms"] < 1e6
+        opts = {k: v for k, v in tuned.items() if k != "estimated_ms"}
+        r = ctx.execute(R[0], H, **opts)
+        assert (r.count, r.checksum) == (want.count, want.checksum)
+        assert r.rows_written == (r.count if mat else 0)
+        ctx.free(r)
