@@ -35,6 +35,7 @@ struct WsCtl {
 };
 
 constexpr uint64_t kMaxClaim = 64;  // units claimed at once by a wave
+constexpr uint64_t kMinClaim = 4;   // ... and at least, in launches of many units per wave
 constexpr uint32_t kFlagSpin = 1u;
 // Frontier steps of 64 children between two progress reports of a working wave: idle waves
 // then see progress while a wave walks a long level alone, not only when a unit or a task ends.
@@ -153,16 +154,23 @@ __device__ __forceinline__ void ws_serve(const WsArgs& A, RunUnits&& run_units,
   for (;;) {
     // busy counts waves holding claimed work; it only keeps idle waves around to help (a wave
     // that leaves early costs parallelism, never work: a pusher always serves its own tasks).
-    // Guided chunks: a quarter of the remaining units per wave, 1 to 64, so a cheap unit does
-    // not pay four shared-counter updates of its own.
+    // Guided chunks of at most 64 units: with 64 or more units per wave, half the remaining
+    // units per wave and at least 4, so cheap units do not pay shared-counter updates of their
+    // own (every claim updates one counter, and the updates serialise: a streamed 1.5 G-row
+    // probe of 23 slices of 1 M units took 316 ms with a quarter per wave and single-unit
+    // claims at the end, 224 ms with half and at least 4, where its PCIe copies alone take
+    // about 215 ms); with fewer (the triejoin's units of 64 roots, heaviest first), a quarter
+    // and at least 1: 4 units at a time took the 4-cliques of R-MAT 20 from 290 to 860 ms.
     uint64_t u = ~0ull, nu = 0;
     if (lane_id() == 0) {
       const unsigned long long seen_u = ld_rlx(&A.ctl->next_unit);
       if (seen_u < A.units) {
         const uint64_t waves = (uint64_t)gridDim.x * kWavesPerBlock;
         const uint64_t most = A.max_claim ? A.max_claim : kMaxClaim;
-        uint64_t k = (A.units - seen_u) / (4 * waves);
-        k = k < 1 ? 1 : (k > most ? most : k);
+        const bool many = A.units >= 64 * waves;
+        const uint64_t least = (many && most >= kMinClaim) ? kMinClaim : 1;
+        uint64_t k = (A.units - seen_u) / ((many ? 2 : 4) * waves);
+        k = k < least ? least : (k > most ? most : k);
         u = add_relaxed(&A.ctl->next_unit, (unsigned long long)k);
         nu = k;
       }
@@ -171,9 +179,13 @@ __device__ __forceinline__ void ws_serve(const WsArgs& A, RunUnits&& run_units,
     nu = lane0_u64(nu);
     if (u < A.units) {
       const uint64_t ue = u + nu < A.units ? u + nu : A.units;
-      if (lane_id() == 0) add_relaxed(&A.ctl->busy, 1u);
+      // busy and progress only serve the waits for tasks: without sharing a wave leaves when
+      // the units run out, and the claim's one counter update is all it pays (the counters
+      // share a cache line, whose updates serialise: 1 M units of a 64 M-row probe slice took
+      // 16.4 ms with three updates per claim, against 6.8 ms for the same rows in one launch)
+      if (A.split && lane_id() == 0) add_relaxed(&A.ctl->busy, 1u);
       run_units(u, ue);
-      if (lane_id() == 0) {
+      if (A.split && lane_id() == 0) {
         add_relaxed(&A.ctl->progress, 1ull);
         add_relaxed(&A.ctl->busy, ~0u);
       }
