@@ -270,17 +270,19 @@ bool fib_eligible(const Plan& P) {
 // Output columns: reuse a parked allocation when it is large enough.
 // Freeing a large buffer and allocating another soon after can cost seconds on this device
 // (the freed memory is cleared before it is handed out again), so released result buffers
-// are parked and reused instead, up to 3/5 of the device memory (a direct-output result of the
-// 4-way chain with 2^32 results reserves 108 GB: with a third of 288 GB as the cap it was freed
-// on release, and each repeat of the query took 3.0 s instead of 65 ms); any allocation that
-// fails gives the parked buffers back and tries again (dmalloc).
+// are parked and reused instead, up to a third of the device memory (more would starve other
+// contexts on the device: a server reserves half of it for its results); any allocation that
+// fails gives the parked buffers back and tries again (dmalloc). A result larger than the cap
+// is freed on release, so repeating such a query pays the clearing each time (a 108 GB direct
+// output of the 4-way chain with 2^32 results: 3.0 s per repeat; 65 ms when parked at a cap of
+// 3/5, which left a server created next to it without room).
 constexpr size_t kPoolCapBuffers = 256;
 
 void pool_put(amj_ctx_t* ctx, void* p, size_t bytes) {
   if (!p) return;
   ctx->out_pool.push_back(amj_ctx::Buf{p, bytes});
   ctx->pool_bytes += bytes;
-  const uint64_t cap_bytes = (uint64_t)ctx->props.totalGlobalMem / 5 * 3;
+  const uint64_t cap_bytes = (uint64_t)ctx->props.totalGlobalMem / 3;
   while (ctx->pool_bytes > cap_bytes || ctx->out_pool.size() > kPoolCapBuffers) {
     const amj_ctx::Buf b = ctx->out_pool.front();
     ctx->out_pool.erase(ctx->out_pool.begin());
