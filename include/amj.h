@@ -103,8 +103,9 @@ typedef struct {
                                    waves report progress as they go, so a slow pass is not
                                    mistaken for a stuck one */
   uint32_t ws_claim_units;      /* ws: most units (64 probe rows or roots each) a wave claims at
-                                   once; claims shrink to a quarter of the remaining units per
-                                   wave (0: 64) */
+                                   once (0: 64); claims shrink with the remaining units: half
+                                   of them per wave and at least 4 units in launches of 64 or
+                                   more units per wave, else a quarter and at least 1 */
   uint32_t reserved[2];
 } amj_opts_t;
 #define AMJ_ENGINE_WAVE 0
