@@ -98,6 +98,7 @@ struct Task {
 
 struct Model {
   uint64_t units, split, cap, stall_ns;
+  uint64_t max_claim = 64;  // WsArgs::max_claim (kMaxClaim when 0 there)
   int workers;
   bool withhold_one;  // a worker reserves a slot and never publishes it (the stuck case)
   std::vector<uint64_t> width;          // children per unit
@@ -196,17 +197,24 @@ struct Model {
       uint64_t u = ~0ull, nu = 0;
       const uint64_t seen_u = ctl.next_unit.load(std::memory_order_relaxed);
       if (seen_u < units) {
-        uint64_t k = (units - seen_u) / (4 * (uint64_t)workers);
-        k = k < 1 ? 1 : (k > 64 ? 64 : k);
+        // guided claims: half the remaining units per worker and at least 4 with 64 or more
+        // units per worker, else a quarter and at least 1; at most max_claim
+        const bool many = units >= 64 * (uint64_t)workers;
+        const uint64_t least = (many && max_claim >= 4) ? 4 : 1;
+        uint64_t k = (units - seen_u) / ((many ? 2 : 4) * (uint64_t)workers);
+        k = k < least ? least : (k > max_claim ? max_claim : k);
         u = ctl.next_unit.fetch_add(k, std::memory_order_relaxed);
         nu = k;
       }
       if (u < units) {
         const uint64_t ue = std::min(u + nu, units);
-        ctl.busy.fetch_add(1, std::memory_order_relaxed);
+        // busy and progress serve only the waits for tasks
+        if (split) ctl.busy.fetch_add(1, std::memory_order_relaxed);
         for (uint64_t v = u; v < ue; ++v) walk(v, 0, width[v], qfull);
-        ctl.progress.fetch_add(1, std::memory_order_relaxed);
-        ctl.busy.fetch_sub(1, std::memory_order_relaxed);
+        if (split) {
+          ctl.progress.fetch_add(1, std::memory_order_relaxed);
+          ctl.busy.fetch_sub(1, std::memory_order_relaxed);
+        }
         continue;
       }
       if (split == 0) break;
@@ -246,8 +254,10 @@ struct Model {
   }
 };
 
-static void run_case(uint64_t units, uint64_t split, uint64_t cap, int workers, uint64_t seed) {
+static void run_case(uint64_t units, uint64_t split, uint64_t cap, int workers, uint64_t seed,
+                     uint64_t max_claim = 64) {
   Model m(units, split, cap, workers, seed, false);
+  m.max_claim = max_claim;
   const uint32_t flags = m.run();
   CHECK(flags == 0, "error flag raised (units %llu split %llu cap %llu workers %d)",
         (unsigned long long)units, (unsigned long long)split, (unsigned long long)cap, workers);
@@ -271,6 +281,10 @@ int main() {
   run_case(3000, 256, 1, 3, 5);        // one slot
   run_case(500, 64, 4096, 1, 6);       // one worker serves its own tasks
   run_case(4000, 192, 100000, 64, 7);  // many more workers than cores
+  run_case(12000, 0, 16, hw, 9);       // 64 or more units per worker: claims of at least 4
+  run_case(12000, 1024, 8192, hw, 10);  // ... with sharing
+  run_case(12000, 0, 16, hw, 11, 2);   // claims capped below the minimum of 4
+  run_case(12000, 256, 8192, hw, 12, 1);  // single-unit claims with sharing
   // a reserved slot whose flag never comes: the taker gives up with the error flag
   Model m(200, 64, 4096, hw, 8, true);
   m.stall_ns = 200'000'000ull;
