@@ -88,11 +88,16 @@ hipError_t launch_lftj_write(const LPlan& P, const uint64_t* unit_base, const LO
                              bool hash, uint64_t* block_sums, uint64_t* n_blocks,
                              const uint64_t* order, hipStream_t s,
                              uint64_t* block_counts = nullptr);
-// Output-size sample: counts[k] = results of the roots (k * per_wave + i) * stride + offset,
-// i < per_wave (per_wave <= 64), for k < waves.
-hipError_t launch_lftj_sample(const LPlan& P, uint64_t* counts, uint64_t waves,
-                              uint32_t per_wave, uint64_t stride, uint64_t offset,
-                              hipStream_t s);
+// Output-size sample over the roots k * stride + offset, k < n, in windows of
+// kLftjSampleWindow candidates for attribute 1: windows[k] = the windows of root k for k < n
+// and windows[n] = 0 (n + 1 entries); then, with woff their exclusive scan (woff[n] = nw),
+// counts[w] = the results of window w (nw entries).
+constexpr uint32_t kLftjSampleWindow = 256;
+hipError_t launch_lftj_sample_widths(const LPlan& P, uint64_t* windows, uint64_t n,
+                                     uint64_t stride, uint64_t offset, hipStream_t s);
+hipError_t launch_lftj_sample_windows(const LPlan& P, const uint64_t* woff, uint64_t n,
+                                      uint64_t nw, uint64_t stride, uint64_t offset,
+                                      uint64_t* counts, hipStream_t s);
 // Direct output: one pass, rows reserved `chunk` at a time (a multiple of 256) with atomics
 // into *rows (zeroed by the caller); rows at or past `capacity` are dropped, so
 // *rows > capacity means the pass must run again. tails[capacity / chunk] (zeroed by the

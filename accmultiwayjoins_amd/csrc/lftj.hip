@@ -78,24 +78,51 @@ __global__ __launch_bounds__(kBlock) void lftj_count_kernel(const LPlan P,
   if (lane_id() == 0) counts[unit] = sink.count;
 }
 
-// Output-size sample: wave k counts the results of the roots (k * per_wave + i) * stride +
-// offset, i < per_wave. A few roots per wave instead of a unit of 64: the launch lasts as long
-// as its slowest wave, and on a skewed graph a unit of 64 roots costs several times its
-// heaviest root.
+// Output-size sample over the roots k * stride + offset, k < n, in two launches. A launch lasts
+// as long as its slowest wave, and a wave that counts a heavy root alone costs that root's
+// whole subtree (6.3 ms for the R-MAT 22 triangle sample of 65,536 roots, 4 per wave), so each
+// sampled root's candidates for attribute 1 are cut into windows of kLftjSampleWindow, counted
+// by one wave each. First: windows[k] = the windows of sampled root k (and windows[n] = 0, so
+// an exclusive scan leaves the total there).
 template <int A, uint64_t SH>
-__global__ __launch_bounds__(kBlock) void lftj_sample_kernel(const LPlan P,
-                                                             uint64_t* __restrict__ counts,
-                                                             uint64_t waves, uint32_t per_wave,
-                                                             uint64_t stride, uint64_t offset) {
-  const uint64_t k = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
-  if (k >= waves) return;  // no barriers in this kernel
-  const uint32_t lane = (uint32_t)lane_id();
-  const uint64_t r = (k * per_wave + lane) * stride + offset;
+__global__ __launch_bounds__(kBlock) void lftj_sample_width_kernel(const LPlan P,
+                                                                   uint64_t* __restrict__ windows,
+                                                                   uint64_t n, uint64_t stride,
+                                                                   uint64_t offset) {
+  const uint64_t k0 = ((uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave) * kWave;
+  if (k0 > n) return;  // wave-uniform; no barriers in this kernel
+  const uint64_t k = k0 + (uint64_t)lane_id();
+  const uint64_t r = k * stride + offset;
   LLv<A> l0;
-  level0_at<A, SH>(P, r, lane < per_wave && r < P.roots, l0);
+  level0_at<A, SH>(P, r, k < n && r < P.roots, l0);
+  if (k < n) windows[k] = (l0.dlen + kLftjSampleWindow - 1) / kLftjSampleWindow;
+  else if (k == n) windows[n] = 0;
+}
+
+// Then wave w counts window j of sampled root k, where woff (the exclusive scan of windows,
+// n + 1 entries) has woff[k] <= w < woff[k + 1] and j = w - woff[k].
+template <int A, uint64_t SH>
+__global__ __launch_bounds__(kBlock) void lftj_sample_window_kernel(
+    const LPlan P, const uint64_t* __restrict__ woff, uint64_t n, uint64_t nw, uint64_t stride,
+    uint64_t offset, uint64_t* __restrict__ counts) {
+  const uint64_t w = (uint64_t)blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+  if (w >= nw) return;  // no barriers in this kernel
+  // the last k < n with woff[k] <= w (woff[0] = 0 <= w); it has a window past w - woff[k]
+  uint64_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const uint64_t mid = lo + (hi - lo) / 2;
+    if (woff[mid] <= w) lo = mid;
+    else hi = mid;
+  }
+  const uint64_t j = w - woff[lo];
+  const uint64_t r = lo * stride + offset;
+  LLv<A> l0;
+  level0_at<A, SH>(P, r, lane_id() == 0 && r < P.roots, l0);
+  const uint64_t c0 = j * kLftjSampleWindow, c1 = c0 + kLftjSampleWindow;
+  restrict_range(l0, (uint32_t)c0, c1 < 0xffffffffull ? (uint32_t)c1 : 0xffffffffu);
   CountSink sink;
   if (l0.T) descend<A, 1, SH>(P, l0, sink, coop_stage<SH>());
-  if (lane == 0) counts[k] = sink.count;
+  if (lane_id() == 0) counts[w] = sink.count;
 }
 
 template <int A, uint64_t SH, int MODE>
@@ -184,14 +211,22 @@ static dim3 grid_of(uint64_t waves) {
   return dim3((unsigned)((waves + kWavesPerBlock - 1) / kWavesPerBlock));
 }
 
-hipError_t launch_lftj_sample(const LPlan& P, uint64_t* counts, uint64_t waves,
-                              uint32_t per_wave, uint64_t stride, uint64_t offset,
-                              hipStream_t s) {
-  if (waves == 0) return hipSuccess;
+hipError_t launch_lftj_sample_widths(const LPlan& P, uint64_t* windows, uint64_t n,
+                                     uint64_t stride, uint64_t offset, hipStream_t s) {
+  const uint64_t waves = n / kWave + 1;  // n + 1 entries, 64 per wave
   return dispatch(P, [&](auto a, auto sh) {
-    hipLaunchKernelGGL((lftj_sample_kernel<decltype(a)::value, decltype(sh)::value>),
-                       grid_of(waves), dim3(kBlock), 0, s, P, counts, waves, per_wave, stride,
-                       offset);
+    hipLaunchKernelGGL((lftj_sample_width_kernel<decltype(a)::value, decltype(sh)::value>),
+                       grid_of(waves), dim3(kBlock), 0, s, P, windows, n, stride, offset);
+  });
+}
+
+hipError_t launch_lftj_sample_windows(const LPlan& P, const uint64_t* woff, uint64_t n,
+                                      uint64_t nw, uint64_t stride, uint64_t offset,
+                                      uint64_t* counts, hipStream_t s) {
+  if (nw == 0) return hipSuccess;
+  return dispatch(P, [&](auto a, auto sh) {
+    hipLaunchKernelGGL((lftj_sample_window_kernel<decltype(a)::value, decltype(sh)::value>),
+                       grid_of(nw), dim3(kBlock), 0, s, P, woff, n, nw, stride, offset, counts);
   });
 }
 

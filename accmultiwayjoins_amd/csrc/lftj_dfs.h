@@ -26,6 +26,17 @@ __device__ __forceinline__ void level0_at(const LPlan& P, uint64_t r, bool valid
   finish(l);
 }
 
+// The entries of `l` cut to candidates [c0, c1) of the flattened level.
+template <int A>
+__device__ __forceinline__ void restrict_range(LLv<A>& l, uint32_t c0, uint32_t c1) {
+  const uint32_t b = l.excl > c0 ? l.excl : c0;
+  const uint32_t e = l.excl + l.dlen < c1 ? l.excl + l.dlen : c1;
+  const uint32_t n = e > b ? e - b : 0u;
+  l.dptr = n ? l.dptr + (b - l.excl) : l.dptr;
+  l.dlen = n;
+  finish(l);
+}
+
 // Level 0 of a unit: P.rpu (0: 64) consecutive level-0 values of driver0, one per lane.
 template <int A, uint64_t SH = 0>
 __device__ __forceinline__ void level0(const LPlan& P, uint64_t unit, LLv<A>& l) {

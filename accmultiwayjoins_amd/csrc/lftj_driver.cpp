@@ -98,7 +98,6 @@ static int make_lplan(amj_ctx_t* ctx, const std::vector<LUse>& uses, const uint3
 }
 
 constexpr uint64_t kLftjSampleRoots = 65536;
-constexpr uint32_t kLftjSampleRootsPerWave = 4;
 
 // DFS with direct output (opts.dro, materialising): the output is sized from the counts of an
 // evenly strided sample of at most kLftjSampleRoots roots (x1.5 + 64 Ki rows; exact when every
@@ -126,10 +125,11 @@ static int unit_order(amj_ctx_t* ctx, const LPlan& P, const uint64_t** order) {
   return AMJ_OK;
 }
 
-// Output size of a direct-output pass from a strided sample of kLftjSampleRoots roots,
-// kLftjSampleRootsPerWave per wave: the launch lasts as long as its slowest wave, and a wave of
-// few roots costs about its heaviest root (a unit of 64 contiguous roots costs several times
-// that on a skewed graph). x1.5 + 64 Ki rows; exact when every root is sampled.
+// Output size of a direct-output pass from a strided sample of kLftjSampleRoots roots, each
+// root's candidates for attribute 1 counted in windows of kLftjSampleWindow by one wave each (a
+// launch lasts as long as its slowest wave: counted whole, 4 roots per wave, the heaviest
+// sampled root of R-MAT 22 held the triangle sample at 6.3 ms). x1.5 + 64 Ki rows; exact when
+// every root is sampled.
 static int sample_capacity(amj_ctx_t* ctx, const LPlan& P, uint64_t* capacity) {
   hipStream_t s = ctx->stream;
   const uint64_t roots = P.roots;
@@ -137,18 +137,27 @@ static int sample_capacity(amj_ctx_t* ctx, const LPlan& P, uint64_t* capacity) {
   if (roots == 0) return AMJ_OK;
   const uint64_t sr = std::min<uint64_t>(roots, kLftjSampleRoots);
   const uint64_t stride = roots / sr;
-  const uint64_t waves = (sr + kLftjSampleRootsPerWave - 1) / kLftjSampleRootsPerWave;
-  void *counts = nullptr, *tmp = nullptr, *sum = nullptr;
-  AMJ_TRY(scratch(ctx, SLOT_UNIT_COUNTS, (waves + 1) * 8, &counts));
-  AMJ_TRY(scratch(ctx, SLOT_SCAN_TEMP, scan_temp_elems(waves + 1) * 8, &tmp));
+  void *woff = nullptr, *counts = nullptr, *tmp = nullptr, *sum = nullptr;
+  AMJ_TRY(scratch(ctx, SLOT_LFTJ_SAMPLE, (sr + 1) * 8, &woff));
   AMJ_TRY(scratch(ctx, SLOT_SUM, 64, &sum));
-  AMJ_HIP_TRY(ctx, launch_lftj_sample(P, (uint64_t*)counts, waves, kLftjSampleRootsPerWave,
-                                      stride, stride / 2, s));
-  AMJ_HIP_TRY(ctx, reduce_sum_u64((const uint64_t*)counts, waves, (uint64_t*)sum,
-                                  (uint64_t*)tmp, s));
-  uint64_t sampled = 0;
-  AMJ_HIP_TRY(ctx, hipMemcpyAsync(&sampled, sum, 8, hipMemcpyDeviceToHost, s));
+  AMJ_TRY(scratch(ctx, SLOT_SCAN_TEMP, scan_temp_elems(sr + 1) * 8, &tmp));
+  AMJ_HIP_TRY(ctx, launch_lftj_sample_widths(P, (uint64_t*)woff, sr, stride, stride / 2, s));
+  AMJ_HIP_TRY(ctx, exclusive_scan_u64((const uint64_t*)woff, (uint64_t*)woff, sr + 1,
+                                      (uint64_t*)tmp, s));
+  uint64_t nw = 0;
+  AMJ_HIP_TRY(ctx, hipMemcpyAsync(&nw, (const uint64_t*)woff + sr, 8, hipMemcpyDeviceToHost, s));
   AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
+  uint64_t sampled = 0;
+  if (nw > 0) {
+    AMJ_TRY(scratch(ctx, SLOT_UNIT_COUNTS, nw * 8, &counts));
+    AMJ_TRY(scratch(ctx, SLOT_SCAN_TEMP, scan_temp_elems(nw) * 8, &tmp));
+    AMJ_HIP_TRY(ctx, launch_lftj_sample_windows(P, (const uint64_t*)woff, sr, nw, stride,
+                                                stride / 2, (uint64_t*)counts, s));
+    AMJ_HIP_TRY(ctx, reduce_sum_u64((const uint64_t*)counts, nw, (uint64_t*)sum,
+                                    (uint64_t*)tmp, s));
+    AMJ_HIP_TRY(ctx, hipMemcpyAsync(&sampled, sum, 8, hipMemcpyDeviceToHost, s));
+    AMJ_HIP_TRY(ctx, hipStreamSynchronize(s));
+  }
   *capacity = sr == roots
                   ? sampled
                   : (uint64_t)((double)sampled * (double)roots / (double)sr * 1.5) + 65536;

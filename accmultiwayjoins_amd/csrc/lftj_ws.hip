@@ -94,17 +94,6 @@ __device__ __forceinline__ void read_ltask(const WsArgs& W, uint64_t slot, int& 
   l.excl = lw[kLwExcl * kWave + lane];
 }
 
-// The entries of `par` cut to candidates [c0, c1) of the flattened level.
-template <int A>
-__device__ __forceinline__ void restrict_range(LLv<A>& l, uint32_t c0, uint32_t c1) {
-  const uint32_t b = l.excl > c0 ? l.excl : c0;
-  const uint32_t e = l.excl + l.dlen < c1 ? l.excl + l.dlen : c1;
-  const uint32_t n = e > b ? e - b : 0u;
-  l.dptr = n ? l.dptr + (b - l.excl) : l.dptr;
-  l.dlen = n;
-  finish(l);
-}
-
 // Levels that the unit engine runs cooperatively for a compiled shape.
 template <int A, int i, uint64_t SH>
 constexpr bool coop_level() {

@@ -96,6 +96,7 @@ enum ScratchSlot {
   SLOT_DYN_TASKS,
   SLOT_DYN_READY,
   SLOT_TUNE_SAMPLE,  // the autotuner's probe samples (the engines it runs use the others)
+  SLOT_LFTJ_SAMPLE,  // the triejoin's output-size sample: window offsets of the sampled roots
 };
 
 int dmalloc(amj_ctx_t* ctx, void** p, size_t bytes);
