@@ -92,7 +92,7 @@ hipError_t launch_lftj_write(const LPlan& P, const uint64_t* unit_base, const LO
 // kLftjSampleWindow candidates for attribute 1: windows[k] = the windows of root k for k < n
 // and windows[n] = 0 (n + 1 entries); then, with woff their exclusive scan (woff[n] = nw),
 // counts[w] = the results of window w (nw entries).
-constexpr uint32_t kLftjSampleWindow = 256;
+constexpr uint32_t kLftjSampleWindow = 64;
 hipError_t launch_lftj_sample_widths(const LPlan& P, uint64_t* windows, uint64_t n,
                                      uint64_t stride, uint64_t offset, hipStream_t s);
 hipError_t launch_lftj_sample_windows(const LPlan& P, const uint64_t* woff, uint64_t n,

@@ -82,7 +82,7 @@ __global__ __launch_bounds__(kBlock) void lftj_count_kernel(const LPlan P,
 // as long as its slowest wave, and a wave that counts a heavy root alone costs that root's
 // whole subtree (6.3 ms for the R-MAT 22 triangle sample of 65,536 roots, 4 per wave), so each
 // sampled root's candidates for attribute 1 are cut into windows of kLftjSampleWindow, counted
-// by one wave each. First: windows[k] = the windows of sampled root k (and windows[n] = 0, so
+// by one wave each (the sum is the same: every sampled root is counted whole). First: windows[k] = the windows of sampled root k (and windows[n] = 0, so
 // an exclusive scan leaves the total there).
 template <int A, uint64_t SH>
 __global__ __launch_bounds__(kBlock) void lftj_sample_width_kernel(const LPlan P,
