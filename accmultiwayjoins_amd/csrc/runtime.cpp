@@ -271,11 +271,11 @@ bool fib_eligible(const Plan& P) {
 // Freeing a large buffer and allocating another soon after can cost seconds on this device
 // (the freed memory is cleared before it is handed out again), so released result buffers
 // are parked and reused instead, up to a third of the device memory (more would starve other
-// contexts on the device: a server reserves half of it for its results); any allocation that
-// fails gives the parked buffers back and tries again (dmalloc). A result larger than the cap
-// is freed on release, so repeating such a query pays the clearing each time (a 108 GB direct
-// output of the 4-way chain with 2^32 results: 3.0 s per repeat; 65 ms when parked at a cap of
-// 3/5, which left a server created next to it without room).
+// contexts on the device: a server reserves half of it for its results), or one buffer alone
+// up to 2/5 of it (a 108 GB direct output of the 4-way chain with 2^32 results took 3.0 s per
+// repeat when it was freed on release, 65 ms when parked); any allocation that fails gives the
+// parked buffers back and tries again (dmalloc). Parking up to 3/5 left a server created next
+// to such a context without room.
 constexpr size_t kPoolCapBuffers = 256;
 
 void pool_put(amj_ctx_t* ctx, void* p, size_t bytes) {
@@ -283,7 +283,13 @@ void pool_put(amj_ctx_t* ctx, void* p, size_t bytes) {
   ctx->out_pool.push_back(amj_ctx::Buf{p, bytes});
   ctx->pool_bytes += bytes;
   const uint64_t cap_bytes = (uint64_t)ctx->props.totalGlobalMem / 3;
-  while (ctx->pool_bytes > cap_bytes || ctx->out_pool.size() > kPoolCapBuffers) {
+  const uint64_t alone_bytes = (uint64_t)ctx->props.totalGlobalMem / 5 * 2;
+  auto over = [&] {
+    if (ctx->out_pool.size() > kPoolCapBuffers) return true;
+    if (ctx->out_pool.size() == 1) return ctx->pool_bytes > alone_bytes;
+    return ctx->pool_bytes > cap_bytes;
+  };
+  while (!ctx->out_pool.empty() && over()) {
     const amj_ctx::Buf b = ctx->out_pool.front();
     ctx->out_pool.erase(ctx->out_pool.begin());
     ctx->pool_bytes -= b.bytes;
