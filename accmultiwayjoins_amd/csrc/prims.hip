@@ -170,24 +170,39 @@ size_t radix_sort_temp_bytes(uint64_t n) {
   return 256 + sizeof(uint32_t) * (hist + 1) + sizeof(uint32_t) * scan_temp_elems(hist) + 256;
 }
 
-// OR over all keys of (key ^ key[0]): the bits in which the keys differ.
+// diff[0] = OR over all keys of (key ^ key[0]): the bits in which the keys differ;
+// diff[1] = 1 when some key is greater than the next (the keys are not in order).
 __global__ __launch_bounds__(kBlock) void key_diff_kernel(const uint64_t* __restrict__ keys,
                                                           uint64_t n, uint64_t* diff) {
   const uint64_t k0 = keys[0];
   uint64_t acc = 0;
+  uint32_t down = 0;
   for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n;
-       i += (uint64_t)gridDim.x * kBlock)
-    acc |= keys[i] ^ k0;
+       i += (uint64_t)gridDim.x * kBlock) {
+    const uint64_t k = keys[i];
+    acc |= k ^ k0;
+    if (i + 1 < n && k > keys[i + 1]) down = 1;
+  }
 #pragma unroll
   for (int d = kWave / 2; d >= 1; d >>= 1) acc |= __shfl_xor(acc, d, kWave);
+  const bool any_down = __ballot(down != 0) != 0ull;
   __shared__ uint64_t part[kWavesPerBlock];
-  if (lane_id() == 0) part[threadIdx.x / kWave] = acc;
+  __shared__ uint32_t part_down[kWavesPerBlock];
+  if (lane_id() == 0) {
+    part[threadIdx.x / kWave] = acc;
+    part_down[threadIdx.x / kWave] = any_down ? 1u : 0u;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
     uint64_t v = 0;
+    uint32_t dn = 0;
 #pragma unroll
-    for (int i = 0; i < kWavesPerBlock; ++i) v |= part[i];
+    for (int i = 0; i < kWavesPerBlock; ++i) {
+      v |= part[i];
+      dn |= part_down[i];
+    }
     if (v) atomicOr((unsigned long long*)diff, (unsigned long long)v);
+    if (dn) atomicOr((unsigned long long*)diff + 1, 1ull);
   }
 }
 
@@ -308,15 +323,22 @@ hipError_t radix_sort_u64(uint64_t* keys, uint64_t* alt, uint64_t n, uint32_t di
   uint32_t* hist = (uint32_t*)(t + 256);
   uint32_t* scan_tmp = hist + hist_n + 1;
 
-  hipError_t e = hipMemsetAsync(diff, 0, sizeof(uint64_t), s);
+  hipError_t e = hipMemsetAsync(diff, 0, 2 * sizeof(uint64_t), s);
   if (e != hipSuccess) return e;
   const unsigned grid = (unsigned)std::min<uint64_t>((n + kBlock - 1) / kBlock, 1024);
   hipLaunchKernelGGL(key_diff_kernel, dim3(grid), dim3(kBlock), 0, s, keys, n, diff);
-  uint64_t hdiff = 0;
-  e = hipMemcpyAsync(&hdiff, diff, sizeof(uint64_t), hipMemcpyDeviceToHost, s);
+  uint64_t hd[2] = {0, 0};
+  e = hipMemcpyAsync(hd, diff, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost, s);
   if (e != hipSuccess) return e;
   e = hipStreamSynchronize(s);
   if (e != hipSuccess) return e;
+  const uint64_t hdiff = hd[0];
+  // Keys already in order whose differing bits all lie in sorted digits: the stable sort is the
+  // identity (an edge list stored as sorted CSR, for instance, skips all its passes).
+  uint64_t sorted_bits = 0;
+  for (int d = 0; d < 8; ++d)
+    if ((digit_mask >> d) & 1u) sorted_bits |= 0xffull << (8 * d);
+  if (hd[1] == 0 && (hdiff & ~sorted_bits) == 0) return hipSuccess;
 
   uint64_t* src = keys;
   uint64_t* dst = alt;
